@@ -1,0 +1,137 @@
+/*
+ * piks3d.h -- C ABI of the MI355X DVM solver (libpiks3d.so).
+ *
+ * The reference (iPACT-Platform/PIKS3D) has no library interface: its hot path
+ * is `subroutine iterate` (src/solver.F90:111) and `subroutine chkConverge`
+ * (src/output.F90:60), called from the driver loop in src/main.F90:95-128 and
+ * operating on module globals set up by src/main.F90:75-89.  This header is the
+ * boundary a host program (the Fortran host in piks3d_amd/fortran, the Python
+ * package, or any ISO_C_BINDING / ctypes caller) binds instead.  Plain C types
+ * only; no exceptions cross it.
+ *
+ * Conventions
+ *  - Every call returns 0 on success or a negative PIKS_E* code; the message
+ *    of the last failure on the calling thread is in piks_last_error().
+ *  - Arrays returned to the host use the reference's layouts so its writers
+ *    and checks can consume them unchanged: cell arrays have Ntotal =
+ *    (Nx+4)(Ny+4)(Nz+4) entries with index l = (i+2) + (j+1)(Nx+4) +
+ *    (k+1)(Nx+4)(Ny+4) - 1 (0-based) for i, j, k in [-1, N+2]
+ *    (src/physicalGrid.F90:704, with Nx already including the fluid layers);
+ *    f is f(Ntotal, Nc8, 8) in Fortran order (src/flow.F90:69,85).
+ *  - A context is not re-entrant: call it from one host thread at a time
+ *    (the reference is MPI_THREAD_FUNNELED, src/main.F90:70).
+ */
+#ifndef PIKS3D_H
+#define PIKS3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIKS_OK 0
+#define PIKS_E_ARG (-1)      /* invalid parameter or input */
+#define PIKS_E_IMAGE (-2)    /* image byte not 0/1, or too short */
+#define PIKS_E_VGRID (-3)    /* velocity grid not provided */
+#define PIKS_E_GEOM (-4)     /* geometry the reference cannot run */
+#define PIKS_E_HIP (-5)      /* HIP runtime failure */
+#define PIKS_E_MEM (-6)      /* device or host memory exhausted / over budget */
+#define PIKS_E_STATE (-7)    /* call not valid in this state */
+
+#define PIKS_PREC_FP64 0       /* f stored and swept in double */
+#define PIKS_PREC_FP32_REFINED 1 /* f32 sweeps inside an f64 defect correction */
+
+/* para.in contents (src/parameters.F90:78-82), after defaults.  nx is the
+ * image width before the fluid layers are added (src/parameters.F90:121). */
+typedef struct piks_params {
+    int32_t nx, ny, nz;            /* &physicalNml Nx, Ny, Nz */
+    int32_t wall_ext_order;        /* &physicalNml wallExtOrder (1, 2 or 3) */
+    int32_t fluid_layer;           /* &physicalNml fluidLayer */
+    double  ref_l;                 /* &physicalNml Ref_L */
+    int32_t nc_fundamental;        /* &velocityNml Nc_fundamental */
+    int32_t half_range;            /* &velocityNml halfRange (0/1) */
+    int32_t chk_converge_step;     /* &solverNml chkConvergeStep */
+    double  kn;                    /* &flowNml Kn */
+    double  press_drop;            /* &flowNml pressDrop */
+    double  accom;                 /* &flowNml accom */
+    /* additions of this build (optional &gpuNml) */
+    int32_t precision;             /* PIKS_PREC_* */
+    int32_t device;                /* HIP device ordinal */
+    int32_t exact_quadrature;      /* 1: exact Gauss-Hermite rules instead of the
+                                      reference's tables (needed for Nc_fundamental=3) */
+    int32_t exact_order;           /* 1: wall-flux and moment sums run serially in the
+                                      reference's order, so every iteration is bit-for-bit
+                                      the reference's (slower; for verification) */
+    int32_t reserved[4];
+} piks_params;
+
+typedef struct piks_info {
+    int32_t nx, ny, nz;            /* domain after fluid layers */
+    int32_t nxt, nyt, nzt;         /* with two ghost layers per side */
+    int64_t ntotal;                /* nxt * nyt * nzt */
+    int32_t nc8;                   /* velocities per octant */
+    int32_t nwall, ncorner;        /* wall cells; 3-fold corners */
+    int64_t nfluid;                /* fluid cells (|mapF|) */
+    int64_t updates_per_iter;      /* nfluid * 8 * nc8 (one update = one cell-velocity pair) */
+    double  ds, mu, diff_flux;     /* grid spacing, sqrt(pi/2)/Kn, 4 sum cz w */
+    double  mass;                  /* last mass-flow value (1 before the first check) */
+    int64_t device_bytes;          /* device memory held by the context */
+    int32_t coupled_walls;         /* walls whose extrapolation reads another wall */
+    int32_t precision;
+} piks_info;
+
+/* Set-up: replaces setupVelocityGrid + setupPhysicalGrid + setupFlow +
+ * allocateBuf (src/main.F90:82-89; src/velocityGrid.F90:90-192,
+ * src/physicalGrid.F90:217-1625, src/flow.F90:76-163).  raw_image holds the
+ * image file's bytes (nx*ny*nz, x fastest; fluid layers are added here).
+ * Fails with PIKS_E_IMAGE on a byte other than 0/1 (src/physicalGrid.F90:268-275),
+ * PIKS_E_VGRID for an unsupported Nc_fundamental (src/velocityGrid.F90:149-153),
+ * PIKS_E_GEOM for a 3-fold corner within two x-layers of the inlet/outlet
+ * (the reference's x self-message then overruns its receive and MPI aborts). */
+int piks_create(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
+                void** ctx_out);
+
+/* nsteps source iterations; replaces `call iterate` (src/solver.F90:111-1940,
+ * called from src/main.F90:100). */
+int piks_iterate(void* ctx, int32_t nsteps);
+
+/* Convergence check; replaces chkConverge (src/output.F90:60-164):
+ * out[0] = mass flow, out[1] = apparent permeability, out[2] = error, and the
+ * stored mass becomes out[0], exactly like the reference. */
+int piks_chk_converge(void* ctx, double out[3]);
+
+/* Mass flow through the plane x = 2 without changing the stored mass
+ * (massLocal of src/output.F90:140-142). */
+int piks_mass_flow(void* ctx, double* mass);
+
+/* Macroscopic fields Rho, Ux, Uy, Uz (src/flow.F90:72), Ntotal doubles each,
+ * as the reference's writers read them (src/output.F90:176-316).  NULL
+ * pointers are skipped. */
+int piks_get_macro(void* ctx, double* rho, double* ux, double* uy, double* uz);
+
+/* f (src/flow.F90:69) as doubles in the reference layout f(Ntotal, Nc8, 8);
+ * fw (src/physicalGrid.F90:192) as fw(nCorner, Nc8, 8). */
+int piks_get_f(void* ctx, double* f);
+int piks_get_fw(void* ctx, double* fw);
+
+/* Cell labels (`image`, src/physicalGrid.F90:124-134), Ntotal int32. */
+int piks_get_labels(void* ctx, int32_t* labels);
+
+int piks_get_info(void* ctx, piks_info* info);
+
+/* Device time of piks_iterate's last step, split by phase (milliseconds):
+ * [0] sweeps, [1] switch commit + exchange + wall BC + inlet/outlet + symmetry,
+ * [2] moments; [3] the whole call (all steps).  HIP events on the context's
+ * stream. */
+int piks_last_timing(void* ctx, double ms[4]);
+
+int piks_destroy(void* ctx);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* piks_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+"""piks3d_amd -- MI355X DVM solver for rarefied gas flow through voxel images.
+
+Python host for libpiks3d.so (C ABI in include/piks3d.h).  The solve loop
+mirrors the reference program (src/main.F90:95-128): iterate, check
+convergence every chkConvergeStep iterations on the mass flow through the
+plane x = 2, stop when error <= eps, report the apparent permeability.
+
+    from piks3d_amd import Solver, read_para, read_raw
+    p = read_para("para.in")
+    s = Solver(p, read_raw(p.image_file_name, p.nx, p.ny, p.nz))
+    result = s.solve()                       # like the reference's main loop
+    rho, ux, uy, uz = s.macro()              # reference cell layout
+
+The native library is required: there is no CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .images import read_raw, write_raw  # noqa: F401
+from .params import Params, parse_para, read_para  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpiks3d.so")
+
+PIKS_PREC_FP64 = 0
+PIKS_PREC_FP32_REFINED = 1
+
+
+class PiksParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("wall_ext_order", ctypes.c_int32), ("fluid_layer", ctypes.c_int32),
+        ("ref_l", ctypes.c_double),
+        ("nc_fundamental", ctypes.c_int32), ("half_range", ctypes.c_int32),
+        ("chk_converge_step", ctypes.c_int32),
+        ("kn", ctypes.c_double), ("press_drop", ctypes.c_double), ("accom", ctypes.c_double),
+        ("precision", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("exact_quadrature", ctypes.c_int32), ("exact_order", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class PiksInfo(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("nxt", ctypes.c_int32), ("nyt", ctypes.c_int32), ("nzt", ctypes.c_int32),
+        ("ntotal", ctypes.c_int64),
+        ("nc8", ctypes.c_int32), ("nwall", ctypes.c_int32), ("ncorner", ctypes.c_int32),
+        ("nfluid", ctypes.c_int64), ("updates_per_iter", ctypes.c_int64),
+        ("ds", ctypes.c_double), ("mu", ctypes.c_double), ("diff_flux", ctypes.c_double),
+        ("mass", ctypes.c_double), ("device_bytes", ctypes.c_int64),
+        ("coupled_walls", ctypes.c_int32), ("precision", ctypes.c_int32),
+    ]
+
+
+# every entry point include/piks3d.h declares, with its ctypes signature
+_API = {
+    "piks_create": (ctypes.c_int, [ctypes.POINTER(PiksParams), ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.POINTER(ctypes.c_void_p)]),
+    "piks_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "piks_chk_converge": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "piks_mass_flow": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "piks_get_macro": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
+    "piks_get_f": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "piks_get_fw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "piks_get_labels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "piks_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PiksInfo)]),
+    "piks_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "piks_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "piks_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class PiksError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("piks error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load libpiks3d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("%s is missing: build it with `python -m piks3d_amd.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _API.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise PiksError(rc, lib().piks_last_error().decode())
+
+
+def to_cparams(p: Params, precision=None, exact_order=False, device=None) -> PiksParams:
+    prec = p.precision if precision is None else precision
+    prec_code = {"fp64": PIKS_PREC_FP64, "fp32": PIKS_PREC_FP32_REFINED}.get(prec, prec)
+    cp = PiksParams()
+    cp.nx, cp.ny, cp.nz = p.nx, p.ny, p.nz
+    cp.wall_ext_order = p.wall_ext_order
+    cp.fluid_layer = p.fluid_layer
+    cp.ref_l = p.ref_l
+    cp.nc_fundamental = p.nc_fundamental
+    cp.half_range = int(bool(p.half_range))
+    cp.chk_converge_step = p.chk_converge_step
+    cp.kn, cp.press_drop, cp.accom = p.kn, p.press_drop, p.accom
+    cp.precision = int(prec_code)
+    cp.device = p.device if device is None else device
+    cp.exact_quadrature = int(bool(p.exact_quadrature))
+    cp.exact_order = int(bool(exact_order))
+    return cp
+
+
+class Solver:
+    """One DVM solve on one GPU (see module docstring)."""
+
+    def __init__(self, params: Params, image, precision=None, exact_order=False, device=None):
+        self.params = params
+        raw = np.ascontiguousarray(np.asarray(image, dtype=np.uint8).ravel())
+        self._cp = to_cparams(params, precision, exact_order, device)
+        h = ctypes.c_void_p()
+        _check(lib().piks_create(ctypes.byref(self._cp), raw.ctypes.data, raw.size, ctypes.byref(h)))
+        self._h = h
+        self.info = self.get_info()
+        self.mass = 1.0
+        self.history = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().piks_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_info(self) -> PiksInfo:
+        info = PiksInfo()
+        _check(lib().piks_get_info(self._h, ctypes.byref(info)))
+        return info
+
+    def iterate(self, n=1):
+        _check(lib().piks_iterate(self._h, int(n)))
+
+    def chk_converge(self):
+        out = (ctypes.c_double * 3)()
+        _check(lib().piks_chk_converge(self._h, out))
+        self.mass = out[0]
+        return out[0], out[1], out[2]
+
+    def mass_flow(self):
+        m = ctypes.c_double()
+        _check(lib().piks_mass_flow(self._h, ctypes.byref(m)))
+        return m.value
+
+    def macro(self):
+        n = self.info.ntotal
+        arrs = [np.empty(n) for _ in range(4)]
+        _check(lib().piks_get_macro(self._h, *[a.ctypes.data for a in arrs]))
+        return tuple(arrs)
+
+    def f(self):
+        i = self.info
+        out = np.empty(8 * i.nc8 * i.ntotal)
+        _check(lib().piks_get_f(self._h, out.ctypes.data))
+        return out.reshape(8, i.nc8, i.ntotal)
+
+    def fw(self):
+        i = self.info
+        out = np.empty(max(i.ncorner, 1) * 8 * i.nc8)
+        _check(lib().piks_get_fw(self._h, out.ctypes.data))
+        return out[: i.ncorner * 8 * i.nc8].reshape(8, i.nc8, i.ncorner)
+
+    def labels(self):
+        out = np.empty(self.info.ntotal, dtype=np.int32)
+        _check(lib().piks_get_labels(self._h, out.ctypes.data))
+        return out
+
+    def timing(self):
+        ms = (ctypes.c_double * 4)()
+        _check(lib().piks_last_timing(self._h, ms))
+        return list(ms)
+
+    def solve(self, max_step=None, chk_step=None, eps=None, callback=None):
+        """The reference's main loop (src/main.F90:95-121): returns a dict with
+        steps, permeability, mass, error and the check history."""
+        p = self.params
+        max_step = p.max_step if max_step is None else max_step
+        chk = p.chk_converge_step if chk_step is None else chk_step
+        eps = p.eps if eps is None else eps
+        error = 1.0
+        perm = 0.0
+        step = 0
+        for step in range(1, max_step + 1):
+            self.iterate(1)
+            if step % chk == 0:
+                m, perm, error = self.chk_converge()
+                self.history.append((step, m, perm, error))
+                if callback:
+                    callback(step, m, perm, error)
+            if error <= eps:
+                if step < chk:
+                    m, perm, error = self.chk_converge()
+                    self.history.append((step, m, perm, error))
+                break
+        return dict(steps=step, permeability=perm, mass=self.mass, error=error,
+                    history=list(self.history))

@@ -1,0 +1,314 @@
+// geometry.cpp -- image -> device layouts (see geometry.h).
+#include "geometry.h"
+
+#include <cstring>
+
+#include "../../include/piks3d.h"
+#include "wallprog.h"
+
+namespace piks {
+
+namespace {
+
+struct Grid {
+  int Nx, Ny, Nz, Nxt;
+  int64_t Nxyt;
+  std::vector<int8_t> a;
+  int8_t& at(int i, int j, int k) {
+    return a[(size_t)((i + 1) + (int64_t)(j + 1) * Nxt + (int64_t)(k + 1) * Nxyt)];
+  }
+};
+
+bool outside(const Grid& g, int i, int j, int k) {
+  return i < 1 || i > g.Nx || j < 1 || j > g.Ny || k < 1 || k > g.Nz;
+}
+
+void set_ghosts(Grid& g, int8_t v) {
+  for (int k = -1; k <= g.Nz + 2; ++k)
+    for (int j = -1; j <= g.Ny + 2; ++j)
+      for (int i = -1; i <= g.Nx + 2; ++i)
+        if (outside(g, i, j, k)) g.at(i, j, k) = v;
+}
+
+// One in-place drilling pass over i = 2..Nx-1 (src/physicalGrid.F90:352-377):
+// a fluid voxel closed on both sides along some axis opens a 2x2x2 block,
+// shifted back at the j = Ny and k = Nz faces.
+void drill(Grid& g) {
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = 2; i <= g.Nx - 1; ++i) {
+        if (g.at(i, j, k) != FLUID) continue;
+        bool closed = (g.at(i + 1, j, k) != FLUID && g.at(i - 1, j, k) != FLUID) ||
+                      (g.at(i, j + 1, k) != FLUID && g.at(i, j - 1, k) != FLUID) ||
+                      (g.at(i, j, k + 1) != FLUID && g.at(i, j, k - 1) != FLUID);
+        if (!closed) continue;
+        int j0 = j == g.Ny ? j - 1 : j;
+        int k0 = k == g.Nz ? k - 1 : k;
+        for (int kk = k0; kk <= k0 + 1; ++kk)
+          for (int jj = j0; jj <= j0 + 1; ++jj)
+            for (int ii = i; ii <= i + 1; ++ii) g.at(ii, jj, kk) = FLUID;
+      }
+}
+
+// One in-place pass turning one-voxel-thick solid into fluid, with the ghost
+// ring counted as fluid (src/physicalGrid.F90:382-422).
+void thin_walls(Grid& g) {
+  set_ghosts(g, FLUID);
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = 1; i <= g.Nx; ++i) {
+        if (g.at(i, j, k) != SOLID) continue;
+        if ((g.at(i + 1, j, k) == FLUID && g.at(i - 1, j, k) == FLUID) ||
+            (g.at(i, j + 1, k) == FLUID && g.at(i, j - 1, k) == FLUID) ||
+            (g.at(i, j, k + 1) == FLUID && g.at(i, j, k - 1) == FLUID))
+          g.at(i, j, k) = FLUID;
+      }
+  set_ghosts(g, GHOST);
+}
+
+constexpr int kSideE = 1, kSideW = 2, kSideN = 4, kSideS = 8, kSideF = 16, kSideB = 32;
+
+}  // namespace
+
+int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geometry& g,
+                   std::string& err) {
+  if (in.nx < 1 || in.ny < 3 || in.nz < 3 || in.fluid_layer < 0) {
+    err = "domain too small: need Nx >= 1, Ny >= 3, Nz >= 3";
+    return PIKS_E_ARG;
+  }
+  const int fl = in.fluid_layer;
+  g.Nx = in.nx + 2 * fl;
+  g.Ny = in.ny;
+  g.Nz = in.nz;
+  if (g.Nx < 4) {
+    err = "domain too small: need Nx + 2*fluidLayer >= 4";
+    return PIKS_E_ARG;
+  }
+  g.Nxt = g.Nx + 4;
+  g.Nyt = g.Ny + 4;
+  g.Nzt = g.Nz + 4;
+  g.Nxyt = (int64_t)g.Nxt * g.Nyt;
+  g.Ntotal = g.Nxyt * g.Nzt;
+  if (g.Ntotal >= (int64_t)1 << 31) {
+    err = "grid too large for 32-bit cell indices";
+    return PIKS_E_ARG;
+  }
+  g.ds = 1.0 / (g.Nx - 1 - 2 * fl);  // src/physicalGrid.F90:228
+
+  const int64_t need = (int64_t)in.nx * in.ny * in.nz;
+  if (nbytes < need) {
+    err = "image has " + std::to_string(nbytes) + " bytes, need " + std::to_string(need);
+    return PIKS_E_IMAGE;
+  }
+  Grid G{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, std::vector<int8_t>((size_t)g.Ntotal, FLUID)};
+  int64_t pos = 0;
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = fl + 1; i <= g.Nx - fl; ++i) {
+        uint8_t v = raw[pos++];
+        if (v > 1) {
+          err = "invalid image data file: byte " + std::to_string((int)v) + " at offset " +
+                std::to_string(pos - 1);
+          return PIKS_E_IMAGE;
+        }
+        G.at(i, j, k) = v ? SOLID : FLUID;
+      }
+  set_ghosts(G, GHOST);
+  drill(G);
+  thin_walls(G);
+  drill(G);
+  thin_walls(G);
+
+  // labels (src/physicalGrid.F90:511-671)
+  g.lab.assign((size_t)g.Ntotal, GHOST);
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = 1; i <= g.Nx; ++i) {
+        int8_t v = G.at(i, j, k);
+        int lab = v;
+        if (v == SOLID) {
+          int p = 1;
+          if (G.at(i + 1, j, k) == FLUID) p *= 2;
+          if (G.at(i - 1, j, k) == FLUID) p *= 3;
+          if (G.at(i, j + 1, k) == FLUID) p *= 5;
+          if (G.at(i, j - 1, k) == FLUID) p *= 7;
+          if (G.at(i, j, k + 1) == FLUID) p *= 9;
+          if (G.at(i, j, k - 1) == FLUID) p *= 11;
+          int l = label_of_product(p);
+          if (l > 0) lab = l;
+        }
+        g.lab[(size_t)g.cell(i, j, k)] = (uint8_t)lab;
+      }
+
+  // walls in raster order, 3-fold corner ids (src/physicalGrid.F90:701-748)
+  g.wid.assign((size_t)g.Ntotal, -1);
+  int near_x = 0;
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = 1; i <= g.Nx; ++i) {
+        int64_t c = g.cell(i, j, k);
+        int lab = g.lab[(size_t)c];
+        if (!is_wall(lab)) continue;
+        g.wid[(size_t)c] = (int32_t)g.wall_cell.size();
+        g.wall_cell.push_back((int32_t)c);
+        g.wall_label.push_back((uint8_t)lab);
+        if (is_corner3(lab)) {
+          g.wall_corner.push_back(g.ncorner++);
+          if (i <= 2 || i >= g.Nx - 1) ++near_x;
+        } else {
+          g.wall_corner.push_back(-1);
+        }
+      }
+  if (near_x > 0) {
+    // the reference's x self-message would carry these corners while the
+    // receive does not (src/flow.F90:124-127 vs src/physicalGrid.F90:835-885):
+    // MPI aborts the reference on such an image
+    err = "3-fold corner within two layers of the x boundary (the reference aborts in MPI): "
+          "add fluid layers";
+    return PIKS_E_GEOM;
+  }
+  const size_t nwall = g.wall_cell.size();
+
+  // extrapolation coefficients (src/physicalGrid.F90:891-948).  The order-3
+  // rule decodes each wall's position from its 1-based local id with the x
+  // index landing one cell east; that decode is reproduced so results match.
+  g.wall_e1.assign(nwall, 2.0);
+  g.wall_e2.assign(nwall, -1.0);
+  if (in.wall_ext_order == 1) {
+    g.wall_e1.assign(nwall, 1.0);
+    g.wall_e2.assign(nwall, 0.0);
+  } else if (in.wall_ext_order != 2) {
+    const int Nxt = g.Nxt;
+    const int Nxy = (int)g.Nxyt;
+    for (size_t w = 0; w < nwall; ++w) {
+      int localid = g.wall_cell[w] + 1;
+      int k = -1 + localid / Nxy;
+      int j = -1 + (localid - (k + 1) * Nxy) / Nxt;
+      int i = -1 + localid - (k + 1) * Nxy - (j + 1) * Nxt;
+      int sides = wall_sides(g.lab[(size_t)g.cell(i, j, k)]);
+      bool first = ((sides & kSideW) && i == 2 && g.lab[(size_t)g.cell(1, j, k)] == FLUID) ||
+                   ((sides & kSideE) && i == g.Nx - 1 && g.lab[(size_t)g.cell(g.Nx, j, k)] == FLUID) ||
+                   ((sides & kSideS) && j == 2 && g.lab[(size_t)g.cell(i, 1, k)] == FLUID) ||
+                   ((sides & kSideN) && j == g.Ny - 1 && g.lab[(size_t)g.cell(i, g.Ny, k)] == FLUID) ||
+                   ((sides & kSideB) && k == 2 && g.lab[(size_t)g.cell(i, j, 1)] == FLUID) ||
+                   ((sides & kSideF) && k == g.Nz - 1 && g.lab[(size_t)g.cell(i, j, g.Nz)] == FLUID);
+      if (first) {
+        g.wall_e1[w] = 1.0;
+        g.wall_e2[w] = 0.0;
+      }
+    }
+  }
+
+  // sweep membership: fluid, interior, and not on the first layer of the
+  // octant's upwind side (src/physicalGrid.F90:977-985 and analogues)
+  auto in_list = [&](int o, int64_t c) -> bool {
+    if (g.lab[(size_t)c] != FLUID) return false;
+    int k = (int)(c / g.Nxyt) - 1;
+    int j = (int)((c % g.Nxyt) / g.Nxt) - 1;
+    int i = (int)(c % g.Nxt) - 1;
+    if (i < 1 || i > g.Nx || j < 1 || j > g.Ny || k < 1 || k > g.Nz) return false;
+    if (osx(o) > 0 ? i < 2 : i > g.Nx - 1) return false;
+    if (osy(o) > 0 ? j < 2 : j > g.Ny - 1) return false;
+    if (osz(o) > 0 ? k < 2 : k > g.Nz - 1) return false;
+    return true;
+  };
+
+  // per wall and octant: which value a fluid cell reading it along y or z
+  // sees.  The reference switches the wall's slot in place when the sweep
+  // reaches the cell just downstream in y or z (src/solver.F90:184-204 ...);
+  // raster order makes x readers see the unswitched slot, y readers the
+  // y-switched one, z readers the z-switched one, else the y-switched one.
+  g.wall_views.assign(nwall, 0);
+  for (size_t w = 0; w < nwall; ++w) {
+    int64_t c = g.wall_cell[w];
+    int lab = g.wall_label[w];
+    uint64_t v = 0;
+    bool any = false;
+    for (int o = 0; o < 8; ++o) {
+      int ys = switch_source(o, 1, lab), zs = switch_source(o, 2, lab);
+      bool ytrig = ys != 0 && in_list(o, c + (int64_t)osy(o) * g.Nxt);
+      bool ztrig = zs != 0 && in_list(o, c + (int64_t)osz(o) * g.Nxyt);
+      uint64_t yc = ytrig ? (uint64_t)ys : 0;
+      uint64_t zc = ztrig ? (uint64_t)zs : yc;
+      v |= (yc << (8 * o)) | (zc << (8 * o + 4));
+      any = any || zc != 0;
+    }
+    g.wall_views[w] = v;
+    if (any) g.commit_walls.push_back((int32_t)w);
+  }
+
+  // per fluid cell: first-order flags and the switched-wall flag per octant
+  // (src/physicalGrid.F90:1254-1622: first order along an axis when the cell
+  // two upstream is a ghost or the upstream cell is a wall facing this one)
+  g.code.assign((size_t)g.Ntotal, 0);
+  const int face_toward[3][2] = {{kSideE, kSideW}, {kSideN, kSideS}, {kSideF, kSideB}};
+  const int64_t stride[3] = {1, g.Nxt, g.Nxyt};
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = 1; i <= g.Nx; ++i) {
+        int64_t c = g.cell(i, j, k);
+        if (g.lab[(size_t)c] != FLUID) continue;
+        ++g.nfluid;
+        bool bnd = i == 1 || i == g.Nx || j == 1 || j == g.Ny || k == 1 || k == g.Nz;
+        (bnd ? g.fluid_boundary : g.fluid_interior).push_back((int32_t)c);
+        uint32_t code = 0;
+        for (int o = 0; o < 8; ++o) {
+          int s[3] = {osx(o), osy(o), osz(o)};
+          bool first[3];
+          for (int a = 0; a < 3; ++a) {
+            int64_t up1 = c - s[a] * stride[a], up2 = c - 2 * s[a] * stride[a];
+            first[a] = g.lab[(size_t)up2] == GHOST ||
+                       (wall_sides(g.lab[(size_t)up1]) & face_toward[a][s[a] > 0 ? 0 : 1]) != 0;
+            if (first[a]) code |= 1u << (3 * o + a);
+          }
+          if (!in_list(o, c)) continue;
+          bool view = false;
+          for (int a = 1; a <= 2; ++a)
+            for (int d = 1; d <= 2; ++d) {
+              if (d == 2 && first[a]) continue;  // zero weight
+              int64_t n = c - d * s[a] * stride[a];
+              int w = g.wid[(size_t)n];
+              if (w < 0) continue;
+              if ((g.wall_views[w] >> (8 * o + (a == 2 ? 4 : 0))) & 15u) view = true;
+            }
+          if (view) code |= 1u << (24 + o);
+        }
+        g.code[(size_t)c] = code;
+      }
+
+  // walls whose extrapolation reads another wall: the reference's serial
+  // wall loop gives them earlier walls' new values and later walls' old ones
+  std::vector<int32_t> snap_of_cell;
+  for (size_t w = 0; w < nwall; ++w) {
+    WallProg P = wall_prog(g.wall_label[w]);
+    int64_t c = g.wall_cell[w];
+    bool coupled = false;
+    int32_t slots[6] = {-1, -1, -1, -1, -1, -1};
+    for (int a = 0; a < P.nface; ++a)
+      for (int d = 1; d <= 2; ++d) {
+        int64_t n = c + (int64_t)d * P.face[a].dir * stride[(int)P.face[a].axis];
+        int wn = g.wid[(size_t)n];
+        if (wn < 0) continue;
+        coupled = true;
+        if ((size_t)wn > w) {  // later in the wall order: needs its old value
+          int idx = -1;
+          for (size_t q = 0; q < g.snap_cell.size(); ++q)
+            if (g.snap_cell[q] == (int32_t)n) idx = (int)q;
+          if (idx < 0) {
+            idx = (int)g.snap_cell.size();
+            g.snap_cell.push_back((int32_t)n);
+          }
+          slots[a * 2 + (d - 1)] = idx;
+        }
+      }
+    if (coupled) {
+      g.coupled.push_back((int32_t)w);
+      for (int q = 0; q < 6; ++q) g.coupled_snap.push_back(slots[q]);
+    } else {
+      g.uncoupled.push_back((int32_t)w);
+    }
+  }
+  return PIKS_OK;
+}
+
+}  // namespace piks

@@ -1,0 +1,62 @@
+// geometry.h -- host preprocessing of the voxel image into the device layouts.
+//
+// Follows src/physicalGrid.F90:217-1625 (single rank): read the 0/1 image into
+// a grid with two ghost layers, drill one-voxel pores and remove one-voxel
+// walls twice, label wall cells by their fluid sides, list walls in raster
+// order with their extrapolation coefficients, and derive per-cell
+// first-order flags of the upwind scheme.  Instead of the reference's eight
+// per-octant raster lists and 9-coefficient arrays it produces what the GPU
+// sweep needs: one byte label per cell, a 32-bit code per fluid cell (three
+// first-order flags per octant plus a "reads a switched wall" flag per
+// octant), and per wall the sources of the reference's in-sweep switches.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace piks {
+
+struct GeomInput {
+  int nx, ny, nz;           // image size as in para.in (nx before fluid layers)
+  int fluid_layer;
+  int wall_ext_order;
+};
+
+struct Geometry {
+  int Nx, Ny, Nz;           // after fluid layers
+  int Nxt, Nyt, Nzt;        // with ghosts
+  int64_t Nxyt, Ntotal;
+  double ds;
+  std::vector<uint8_t> lab;         // Ntotal labels
+  std::vector<uint32_t> code;       // Ntotal; fluid cells only are meaningful
+  // walls in the reference's vecWall order
+  std::vector<int32_t> wall_cell;
+  std::vector<uint8_t> wall_label;
+  std::vector<double> wall_e1, wall_e2;
+  std::vector<int32_t> wall_corner; // 3-fold corner id (0-based) or -1
+  std::vector<uint64_t> wall_views; // 8 octants x (y code, z code) nibbles
+  std::vector<int32_t> wid;         // Ntotal: wall index or -1
+  int ncorner = 0;
+  // walls whose extrapolation reads another wall (order-dependent in the
+  // reference), in vecWall order, with per (face, distance) snapshot slots
+  std::vector<int32_t> coupled;           // wall indices
+  std::vector<int32_t> coupled_snap;      // 6 per coupled wall: snapshot index or -1
+  std::vector<int32_t> snap_cell;         // cells to snapshot before the wall pass
+  std::vector<int32_t> uncoupled;         // wall indices processed in parallel
+  // fluid cells
+  std::vector<int32_t> fluid_interior;    // not on any boundary plane: moments from sweep partials
+  std::vector<int32_t> fluid_boundary;    // on i=1,Nx / j=1,Ny / k=1,Nz planes
+  std::vector<int32_t> commit_walls;      // walls with a switch in some octant
+  int64_t nfluid = 0;
+
+  int64_t cell(int i, int j, int k) const {
+    return (int64_t)(i + 1) + (int64_t)(j + 1) * Nxt + (int64_t)(k + 1) * Nxyt;
+  }
+};
+
+// Returns 0 on success; on failure returns a PIKS_E* code and fills err.
+int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geometry& g,
+                   std::string& err);
+
+}  // namespace piks

@@ -1,0 +1,1018 @@
+// piks3d.hip -- MI355X (gfx950) DVM iteration for linearized BGK flow through a
+// voxel image, behind the C ABI in include/piks3d.h.
+//
+// One source iteration of the reference (src/solver.F90:111-1940) becomes:
+//   sweep      8 octants at once; hyperplane wavefront over tiles, one launch
+//              per tile diagonal; a 64-lane wave updates all velocities of one
+//              cell (f loads coalesced along the velocity index); partial
+//              moments of each (cell, octant) reduced in registers
+//   commit     the reference's in-sweep wall switches, applied after the sweep
+//              (readers were redirected to the switched sources during it)
+//   halo       periodic x exchange with the reference's one-iteration lag
+//   wall BC    one wave per wall: extrapolate, reduce the wall flux over the
+//              velocities, re-emit (diffuse + specular)
+//   in/out     pressure-drop inlet/outlet, then the four symmetry planes
+//   moments    sum of the sweep partials; recomputed from f on boundary planes
+// Device layout: f[octant][cell][velocity] over the full ghosted grid, cell =
+// (i+1) + (j+1)Nxt + (k+1)Nxt*Nyt, velocity fastest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/piks3d.h"
+#include "gauss_hermite_tables.h"
+#include "geometry.h"
+#include "wallprog.h"
+
+using namespace piks;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(PIKS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+  int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
+struct Dims {
+  int Nx, Ny, Nz, Nxt;
+  int64_t Nxyt, Ntotal;
+  int Nc8;
+};
+
+// ------------------------------------------------------------------ sweep
+
+template <typename T>
+struct SweepArgs {
+  T* f;
+  const uint8_t* lab;
+  const uint32_t* code;
+  const double* rho;
+  const double* ux;
+  const double* uy;
+  const double* uz;
+  double* part;            // [cell][octant][4] partial moments; null = skip
+  const int32_t* wid;
+  const uint64_t* views;
+  const int32_t* wall_corner;
+  const T* fw;             // [corner][octant][velocity]
+  const double* cx;
+  const double* cy;
+  const double* cz;
+  const double* w;
+  double mu, c0, c1, c2, d1;  // 1.5/ds, 2/ds, -0.5/ds, 1/ds
+  Dims d;
+  int TX, TY, TZ;
+  const int4* tiles;       // tiles of this diagonal (oriented tile coords)
+  const uint16_t* hcell;   // in-tile cells grouped by hyperplane
+  const int32_t* hoff;     // nh + 1 offsets
+  int nh;
+};
+
+// Value a cell reading neighbour n along `axis` (1 = y, 2 = z) sees in octant
+// o, when the reference would have switched that wall slot by then; null if
+// the plain f slot applies.
+template <typename T>
+__device__ __forceinline__ const T* switched_source(const SweepArgs<T>& a, int64_t n, int o,
+                                                    int axis) {
+  if (a.lab[n] < 20) return nullptr;
+  const int w = a.wid[n];
+  const int src = (int)((a.views[w] >> (8 * o + (axis == 2 ? 4 : 0))) & 15u);
+  if (src == 0) return nullptr;
+  if (src <= 8) return a.f + ((size_t)(src - 1) * a.d.Ntotal + n) * a.d.Nc8;
+  return a.fw + ((size_t)a.wall_corner[w] * 8 + o) * a.d.Nc8;
+}
+
+// One update of f(cell, l, o), src/solver.F90:206-214 and analogues, with the
+// same operations in the same order (built with -ffp-contract=off): signs are
+// applied as exact negations, zero-weight second-order terms are skipped.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
+  const int o = blockIdx.x & 7;
+  const int4 tile = a.tiles[blockIdx.x >> 3];
+  const int sx = osx(o), sy = osy(o), sz = osz(o);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nwaves = blockDim.x / kWave;
+  const int64_t dx = sx, dy = (int64_t)sy * a.d.Nxt, dz = (int64_t)sz * a.d.Nxyt;
+  const int Nc8 = a.d.Nc8;
+  T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8;
+
+  for (int h = 0; h < a.nh; ++h) {
+    for (int q = a.hoff[h] + wave; q < a.hoff[h + 1]; q += nwaves) {
+      const int packed = a.hcell[q];
+      const int oi = tile.x * a.TX + (packed & 31);
+      const int oj = tile.y * a.TY + ((packed >> 5) & 31);
+      const int ok = tile.z * a.TZ + (packed >> 10);
+      // oriented coordinate 0 is the octant's inflow layer: not swept
+      if (oi < 1 || oj < 1 || ok < 1 || oi >= a.d.Nx || oj >= a.d.Ny || ok >= a.d.Nz) continue;
+      const int i = sx > 0 ? 1 + oi : a.d.Nx - oi;
+      const int j = sy > 0 ? 1 + oj : a.d.Ny - oj;
+      const int k = sz > 0 ? 1 + ok : a.d.Nz - ok;
+      const int64_t cell = uniform64((int64_t)(i + 1) + (int64_t)(j + 1) * a.d.Nxt +
+                                     (int64_t)(k + 1) * a.d.Nxyt);
+      if (a.lab[cell] != FLUID) continue;
+      const uint32_t cc = a.code[cell];
+      const bool fx = (cc >> (3 * o)) & 1u, fy = (cc >> (3 * o + 1)) & 1u,
+                 fz = (cc >> (3 * o + 2)) & 1u;
+      const bool view = (cc >> (24 + o)) & 1u;
+      const double r = a.rho[cell], u = a.ux[cell], v = a.uy[cell], ww = a.uz[cell];
+      const double C0x = fx ? a.d1 : a.c0, C1x = fx ? a.d1 : a.c1;
+      const double C0y = fy ? a.d1 : a.c0, C1y = fy ? a.d1 : a.c1;
+      const double C0z = fz ? a.d1 : a.c0, C1z = fz ? a.d1 : a.c1;
+      const T* py1 = fo + (cell - dy) * Nc8;
+      const T* py2 = fo + (cell - 2 * dy) * Nc8;
+      const T* pz1 = fo + (cell - dz) * Nc8;
+      const T* pz2 = fo + (cell - 2 * dz) * Nc8;
+      if (view) {
+        const T* s;
+        if ((s = switched_source(a, cell - dy, o, 1))) py1 = s;
+        if (!fy && (s = switched_source(a, cell - 2 * dy, o, 1))) py2 = s;
+        if ((s = switched_source(a, cell - dz, o, 2))) pz1 = s;
+        if (!fz && (s = switched_source(a, cell - 2 * dz, o, 2))) pz2 = s;
+      }
+      double pr = 0.0, pu = 0.0, pv = 0.0, pw = 0.0;
+      for (int l = lane; l < Nc8; l += kWave) {
+        const double cxl = a.cx[l], cyl = a.cy[l], czl = a.cz[l];
+        double t = sx > 0 ? cxl * u : -(cxl * u);
+        t = sy > 0 ? t + cyl * v : t - cyl * v;
+        t = sz > 0 ? t + czl * ww : t - czl * ww;
+        const double feq = a.w[l] * (r + 1.0 * t);
+        T* pc = fo + cell * Nc8 + l;
+        double num = a.mu * (feq - 0.5 * (double)(*pc));
+        num = num + cxl * C1x * (double)fo[(cell - dx) * Nc8 + l];
+        if (!fx) num = num + cxl * a.c2 * (double)fo[(cell - 2 * dx) * Nc8 + l];
+        num = num + cyl * C1y * (double)py1[l];
+        if (!fy) num = num + cyl * a.c2 * (double)py2[l];
+        num = num + czl * C1z * (double)pz1[l];
+        if (!fz) num = num + czl * a.c2 * (double)pz2[l];
+        double den = 0.5 * a.mu + cxl * C0x;
+        den = den + cyl * C0y;
+        den = den + czl * C0z;
+        const T fnew = (T)(num / den);
+        *pc = fnew;
+        pr += (double)fnew;
+        pu += cxl * (double)fnew;
+        pv += cyl * (double)fnew;
+        pw += czl * (double)fnew;
+      }
+      if (a.part) {
+        pr = wave_sum(pr);
+        pu = wave_sum(pu);
+        pv = wave_sum(pv);
+        pw = wave_sum(pw);
+        if (lane < 4) {
+          const double val = lane == 0 ? pr : lane == 1 ? sx * pu : lane == 2 ? sy * pv : sz * pw;
+          a.part[((size_t)cell * 8 + o) * 4 + lane] = val;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ switch commit
+
+// Final state of the switched wall slots after a sweep: the z switch if it
+// fired, else the y switch (src/solver.F90:184-204 and analogues).
+template <typename T>
+__global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
+                         const int32_t* wall_cell, const int32_t* wall_corner,
+                         const uint64_t* views, Dims d) {
+  const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (idx >= nwalls) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = walls[idx];
+  const int64_t cell = wall_cell[w];
+  const uint64_t vv = views[w];
+  for (int o = 0; o < 8; ++o) {
+    const int src = (int)((vv >> (8 * o + 4)) & 15u);
+    if (src == 0) continue;
+    const T* s = src <= 8 ? f + ((size_t)(src - 1) * d.Ntotal + cell) * d.Nc8
+                          : fw + ((size_t)wall_corner[w] * 8 + o) * d.Nc8;
+    T* dst = f + ((size_t)o * d.Ntotal + cell) * d.Nc8;
+    for (int l = lane; l < d.Nc8; l += kWave) dst[l] = s[l];
+  }
+}
+
+// ------------------------------------------------------------------ x exchange
+
+// Ghost plane x = 0 receives plane x = Nx and ghost x = Nx + 1 receives plane
+// x = 1, both as they were after the previous iteration's sweep (the
+// reference's self send/receive posts last iteration's packed buffers,
+// src/solver.F90:144-171, 535-560).  y/z ghost rows stay zero (MPI_PROC_NULL).
+template <typename T>
+__global__ void k_halo(T* f, T* buf_e, T* buf_w, Dims d) {
+  const int64_t n = (int64_t)d.Ny * d.Nz * 8 * d.Nc8;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(t % d.Nc8);
+    int64_t r = t / d.Nc8;
+    const int o = (int)(r % 8);
+    r /= 8;
+    const int j = 1 + (int)(r % d.Ny);
+    const int k = 1 + (int)(r / d.Ny);
+    const int64_t row = (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+    T* fo = f + (size_t)o * d.Ntotal * d.Nc8;
+    const int64_t east = row + d.Nx + 1, west = row + 2;   // x = Nx, x = 1
+    const int64_t g0 = row + 1, gN = row + d.Nx + 2;       // x = 0, x = Nx + 1
+    const T ve = fo[east * d.Nc8 + l], vw = fo[west * d.Nc8 + l];
+    fo[g0 * d.Nc8 + l] = buf_e[t];
+    fo[gN * d.Nc8 + l] = buf_w[t];
+    buf_e[t] = ve;
+    buf_w[t] = vw;
+  }
+}
+
+// ------------------------------------------------------------------ wall BC
+
+template <typename T>
+struct WallArgs {
+  T* f;
+  T* fw;
+  const int32_t* list;      // wall indices to process
+  int n;
+  const int32_t* wall_cell;
+  const uint8_t* wall_label;
+  const double* e1;
+  const double* e2;
+  const int32_t* wall_corner;
+  const double* cx;
+  const double* cy;
+  const double* cz;
+  const double* w;
+  double accom, inv_diff_flux, diff_flux;
+  Dims d;
+  int exact;                // sum the wall flux serially in the reference's order
+  // coupled walls: per wall 6 snapshot indices (face, distance) or -1
+  const int32_t* snap_idx;
+  const T* snap;            // [snapshot][octant][velocity]
+};
+
+template <typename T, bool COUPLED>
+__device__ __forceinline__ double load_nbr(const WallArgs<T>& a, int64_t cell, int o, int l,
+                                           int slot) {
+  if (COUPLED && slot >= 0) return (double)a.snap[((size_t)slot * 8 + o) * a.d.Nc8 + l];
+  return (double)a.f[((size_t)o * a.d.Ntotal + cell) * a.d.Nc8 + l];
+}
+
+// src/solver.F90:772-1823: one wall, all velocities, on one wave.
+template <int L, typename T, bool COUPLED>
+__device__ void wall_one(const WallArgs<T>& a, int idx, int lane) {
+  constexpr WallProg P = wall_prog(L);
+  constexpr uint32_t written = wall_prog_written(P);
+  const int wall = a.list[idx];
+  const int64_t cell = a.wall_cell[wall];
+  const int corner = a.wall_corner[wall];
+  const double e1 = a.e1[wall], e2 = a.e2[wall];
+  const int Nc8 = a.d.Nc8;
+  const int64_t stride[3] = {1, a.d.Nxt, a.d.Nxyt};
+  int sidx[6] = {-1, -1, -1, -1, -1, -1};
+  if (COUPLED)
+    for (int q = 0; q < 6; ++q) sidx[q] = a.snap_idx[idx * 6 + q];
+  const double* cvel[3] = {a.cx, a.cy, a.cz};
+
+  // wall flux per face: R_a = sum_l c_a(l) (v0 + v1 + v2 + v3) / DiffFlux
+  double R[3] = {0.0, 0.0, 0.0};
+  if (!a.exact) {
+    for (int l = lane; l < Nc8; l += kWave) {
+#pragma unroll
+      for (int fa = 0; fa < P.nface; ++fa) {
+        const int64_t st = P.face[fa].dir * stride[P.face[fa].axis];
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int oo = P.face[fa].in[q];
+          const double val = e1 * load_nbr<T, COUPLED>(a, cell + st, oo, l, sidx[fa * 2]) +
+                             e2 * load_nbr<T, COUPLED>(a, cell + 2 * st, oo, l, sidx[fa * 2 + 1]);
+          s = q == 0 ? val : s + val;
+        }
+        R[fa] = R[fa] + cvel[P.face[fa].axis][l] * s;
+      }
+    }
+#pragma unroll
+    for (int fa = 0; fa < P.nface; ++fa) R[fa] = wave_sum(R[fa]) / a.diff_flux;
+  } else {
+    if (lane == 0) {
+      for (int l = 0; l < Nc8; ++l) {
+#pragma unroll
+        for (int fa = 0; fa < P.nface; ++fa) {
+          const int64_t st = P.face[fa].dir * stride[P.face[fa].axis];
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int oo = P.face[fa].in[q];
+            const double val = e1 * load_nbr<T, COUPLED>(a, cell + st, oo, l, sidx[fa * 2]) +
+                               e2 * load_nbr<T, COUPLED>(a, cell + 2 * st, oo, l, sidx[fa * 2 + 1]);
+            s = q == 0 ? val : s + val;
+          }
+          R[fa] = R[fa] + cvel[P.face[fa].axis][l] * s;
+        }
+      }
+    }
+#pragma unroll
+    for (int fa = 0; fa < P.nface; ++fa) R[fa] = __shfl(R[fa], 0, kWave) / a.diff_flux;
+  }
+
+  // re-emission, in the reference's statement order
+  for (int l = lane; l < Nc8; l += kWave) {
+    double v[24];
+#pragma unroll
+    for (int fa = 0; fa < P.nface; ++fa) {
+      const int64_t st = P.face[fa].dir * stride[P.face[fa].axis];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int oo = P.face[fa].in[q];
+        v[P.face[fa].store + oo] = e1 * load_nbr<T, COUPLED>(a, cell + st, oo, l, sidx[fa * 2]) +
+                                   e2 * load_nbr<T, COUPLED>(a, cell + 2 * st, oo, l, sidx[fa * 2 + 1]);
+      }
+    }
+    const double aw = a.accom * a.w[l], b = 1.0 - a.accom;
+#pragma unroll
+    for (int t = 0; t < P.nwr; ++t) v[P.wr[t].dst] = aw * R[P.wr[t].rho] + b * v[P.wr[t].src];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      if (!((written >> s) & 1u)) continue;
+      if (s < 8)
+        a.f[((size_t)s * a.d.Ntotal + cell) * Nc8 + l] = (T)v[s];
+      else
+        a.fw[((size_t)corner * 8 + (s - 8)) * Nc8 + l] = (T)v[s];
+    }
+  }
+}
+
+template <typename T, bool COUPLED>
+__device__ void wall_dispatch(const WallArgs<T>& a, int idx, int lane) {
+  switch (a.wall_label[a.list[idx]]) {
+#define PK_CASE(L) case L: wall_one<L, T, COUPLED>(a, idx, lane); break;
+    PK_CASE(L_E) PK_CASE(L_W) PK_CASE(L_N) PK_CASE(L_S) PK_CASE(L_F) PK_CASE(L_B)
+    PK_CASE(L_EN) PK_CASE(L_WN) PK_CASE(L_ES) PK_CASE(L_WS)
+    PK_CASE(L_NF) PK_CASE(L_NB) PK_CASE(L_SF) PK_CASE(L_SB)
+    PK_CASE(L_EF) PK_CASE(L_EB) PK_CASE(L_WF) PK_CASE(L_WB)
+    PK_CASE(L_ENF) PK_CASE(L_WNF) PK_CASE(L_WSF) PK_CASE(L_ESF)
+    PK_CASE(L_ENB) PK_CASE(L_WNB) PK_CASE(L_WSB) PK_CASE(L_ESB)
+#undef PK_CASE
+    default: break;
+  }
+}
+
+// walls that read no other wall: all at once, one wave each
+template <typename T>
+__global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
+  const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (idx >= a.n) return;
+  wall_dispatch<T, false>(a, idx, threadIdx.x & (kWave - 1));
+}
+
+// walls that read another wall: one wave, in the reference's wall order
+template <typename T>
+__global__ __launch_bounds__(64) void k_walls_serial(WallArgs<T> a) {
+  for (int idx = 0; idx < a.n; ++idx) {
+    wall_dispatch<T, true>(a, idx, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void k_snapshot(const T* f, T* snap, const int32_t* cells, int n, Dims d) {
+  const int64_t tot = (int64_t)n * 8 * d.Nc8;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(t % d.Nc8);
+    const int o = (int)((t / d.Nc8) % 8);
+    const int s = (int)(t / (8 * d.Nc8));
+    snap[t] = f[((size_t)o * d.Ntotal + cells[s]) * d.Nc8 + l];
+  }
+}
+
+// ------------------------------------------------------------------ inlet/outlet, symmetry
+
+// src/solver.F90:1832-1862: f(1) = f(0) + w dp for octants with cx > 0 and
+// f(Nx) = f(Nx+1) - w dp for cx < 0, on every interior (j, k) row.
+template <typename T>
+__global__ void k_inout(T* f, const double* w, double pd, Dims d) {
+  const int64_t n = (int64_t)d.Ny * d.Nz * 4 * d.Nc8;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(t % d.Nc8);
+    int64_t r = t / d.Nc8;
+    const int q = (int)(r % 4);
+    r /= 4;
+    const int j = 1 + (int)(r % d.Ny);
+    const int k = 1 + (int)(r / d.Ny);
+    const int64_t row = (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+    const int oin = q == 0 ? 0 : q == 1 ? 3 : q == 2 ? 4 : 7;
+    const int oout = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 5 : 6;
+    T* fi = f + (size_t)oin * d.Ntotal * d.Nc8;
+    T* fo = f + (size_t)oout * d.Ntotal * d.Nc8;
+    const double wpd = w[l] * pd;
+    fi[(row + 2) * d.Nc8 + l] = (T)((double)fi[(row + 1) * d.Nc8 + l] + wpd);
+    fo[(row + d.Nx + 1) * d.Nc8 + l] = (T)((double)fo[(row + d.Nx + 2) * d.Nc8 + l] - wpd);
+  }
+}
+
+// src/solver.F90:1867-1892: y = 1 (south) then y = Ny (north) mirror copies.
+template <typename T>
+__global__ void k_sym_y(T* f, Dims d) {
+  const int64_t n = (int64_t)d.Nx * d.Nz * d.Nc8;
+  const size_t os = (size_t)d.Ntotal * d.Nc8;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(t % d.Nc8);
+    const int i = 1 + (int)((t / d.Nc8) % d.Nx);
+    const int k = 1 + (int)(t / ((int64_t)d.Nc8 * d.Nx));
+    const size_t s = ((size_t)(i + 1) + 2 * (size_t)d.Nxt + (size_t)(k + 1) * d.Nxyt) * d.Nc8 + l;
+    f[1 * os + s] = f[2 * os + s]; f[0 * os + s] = f[3 * os + s];
+    f[5 * os + s] = f[6 * os + s]; f[4 * os + s] = f[7 * os + s];
+    const size_t q = ((size_t)(i + 1) + (size_t)(d.Ny + 1) * d.Nxt + (size_t)(k + 1) * d.Nxyt) * d.Nc8 + l;
+    f[2 * os + q] = f[1 * os + q]; f[3 * os + q] = f[0 * os + q];
+    f[6 * os + q] = f[5 * os + q]; f[7 * os + q] = f[4 * os + q];
+  }
+}
+
+// src/solver.F90:1893-1918: z = 1 (back) then z = Nz (front).
+template <typename T>
+__global__ void k_sym_z(T* f, Dims d) {
+  const int64_t n = (int64_t)d.Nx * d.Ny * d.Nc8;
+  const size_t os = (size_t)d.Ntotal * d.Nc8;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(t % d.Nc8);
+    const int i = 1 + (int)((t / d.Nc8) % d.Nx);
+    const int j = 1 + (int)(t / ((int64_t)d.Nc8 * d.Nx));
+    const size_t s = ((size_t)(i + 1) + (size_t)(j + 1) * d.Nxt + 2 * (size_t)d.Nxyt) * d.Nc8 + l;
+    f[0 * os + s] = f[4 * os + s]; f[1 * os + s] = f[5 * os + s];
+    f[2 * os + s] = f[6 * os + s]; f[3 * os + s] = f[7 * os + s];
+    const size_t q = ((size_t)(i + 1) + (size_t)(j + 1) * d.Nxt + (size_t)(d.Nz + 1) * d.Nxyt) * d.Nc8 + l;
+    f[4 * os + q] = f[0 * os + q]; f[5 * os + q] = f[1 * os + q];
+    f[6 * os + q] = f[2 * os + q]; f[7 * os + q] = f[3 * os + q];
+  }
+}
+
+// ------------------------------------------------------------------ moments
+
+// Interior fluid cells: their f is final after the sweep, so the moments are
+// the sum of the eight octant partials.
+__global__ void k_moments_partials(const double* part, const int32_t* cells, int64_t n,
+                                   double* rho, double* ux, double* uy, double* uz) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = cells[t];
+    const double* p = part + (size_t)c * 32;
+    double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      r += p[o * 4 + 0];
+      x += p[o * 4 + 1];
+      y += p[o * 4 + 2];
+      z += p[o * 4 + 3];
+    }
+    rho[c] = r; ux[c] = x; uy[c] = y; uz[c] = z;
+  }
+}
+
+// Moments straight from f (src/solver.F90:1925-1937), one wave per cell.
+template <typename T>
+__global__ __launch_bounds__(256) void k_moments_direct(const T* f, const int32_t* cells, int64_t n,
+                                                        const double* cx, const double* cy,
+                                                        const double* cz, Dims d, double* rho,
+                                                        double* ux, double* uy, double* uz) {
+  const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (idx >= n) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t c = cells[idx];
+  const size_t os = (size_t)d.Ntotal * d.Nc8;
+  double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+  for (int l = lane; l < d.Nc8; l += kWave) {
+    const size_t b = (size_t)c * d.Nc8 + l;
+    const double f1 = f[b], f2 = f[os + b], f3 = f[2 * os + b], f4 = f[3 * os + b];
+    const double f5 = f[4 * os + b], f6 = f[5 * os + b], f7 = f[6 * os + b], f8 = f[7 * os + b];
+    r += f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
+    x += cx[l] * (f1 - f2 - f3 + f4 + f5 - f6 - f7 + f8);
+    y += cy[l] * (f1 + f2 - f3 - f4 + f5 + f6 - f7 - f8);
+    z += cz[l] * (f1 + f2 + f3 + f4 - f5 - f6 - f7 - f8);
+  }
+  r = wave_sum(r); x = wave_sum(x); y = wave_sum(y); z = wave_sum(z);
+  if (lane == 0) { rho[c] = r; ux[c] = x; uy[c] = y; uz[c] = z; }
+}
+
+// The same in the reference's summation order (one thread per cell), for
+// bit-for-bit comparison with the oracle.
+template <typename T>
+__global__ void k_moments_serial(const T* f, const int32_t* cells, int64_t n, const double* cx,
+                                 const double* cy, const double* cz, Dims d, double* rho,
+                                 double* ux, double* uy, double* uz) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t c = cells[t];
+  const size_t os = (size_t)d.Ntotal * d.Nc8;
+  double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+  for (int l = 0; l < d.Nc8; ++l) {
+    const size_t b = (size_t)c * d.Nc8 + l;
+    const double f1 = f[b], f2 = f[os + b], f3 = f[2 * os + b], f4 = f[3 * os + b];
+    const double f5 = f[4 * os + b], f6 = f[5 * os + b], f7 = f[6 * os + b], f8 = f[7 * os + b];
+    r = r + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
+    x = x + cx[l] * (f1 - f2 - f3 + f4 + f5 - f6 - f7 + f8);
+    y = y + cy[l] * (f1 + f2 - f3 - f4 + f5 + f6 - f7 - f8);
+    z = z + cz[l] * (f1 + f2 + f3 + f4 - f5 - f6 - f7 - f8);
+  }
+  rho[c] = r; ux[c] = x; uy[c] = y; uz[c] = z;
+}
+
+__global__ void k_gather_plane(const double* ux, double* out, Dims d, int column) {
+  const int64_t n = (int64_t)d.Ny * d.Nz;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = 1 + (int)(t % d.Ny), k = 1 + (int)(t / d.Ny);
+    out[t] = ux[(int64_t)(column + 1) + (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt];
+  }
+}
+
+// ------------------------------------------------------------------ context
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+struct Ctx {
+  piks_params prm{};
+  Geometry g;
+  Dims d{};
+  int nc_fund = 0;
+  std::vector<double> cx, cy, cz, w;
+  double mu = 0, diff_flux = 0, mass = 1.0;
+  int exact = 0;         // serial reductions in the reference's order
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int64_t bytes = 0;
+  // device state
+  double* f = nullptr;
+  double* fw = nullptr;
+  double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
+  double* part = nullptr;
+  double *buf_e = nullptr, *buf_w = nullptr;
+  double* snap = nullptr;
+  double* plane = nullptr;
+  double *dcx = nullptr, *dcy = nullptr, *dcz = nullptr, *dw = nullptr;
+  uint8_t* lab = nullptr;
+  uint32_t* code = nullptr;
+  int32_t* wid = nullptr;
+  uint64_t* views = nullptr;
+  int32_t *wall_cell = nullptr, *wall_corner = nullptr;
+  uint8_t* wall_label = nullptr;
+  double *e1 = nullptr, *e2 = nullptr;
+  int32_t *uncoupled = nullptr, *coupled = nullptr, *coupled_snap = nullptr, *snap_cell = nullptr;
+  int32_t *commit_walls = nullptr;
+  int32_t *fluid_int = nullptr, *fluid_bnd = nullptr, *fluid_all = nullptr;
+  // tiles
+  int TX = 8, TY = 8, TZ = 8;
+  int ndiag = 0;
+  std::vector<int> diag_off;   // ndiag + 1
+  int4* tiles = nullptr;
+  uint16_t* hcell = nullptr;
+  int32_t* hoff = nullptr;
+  int nh = 0;
+  // timing
+  hipEvent_t ev[5] = {};
+  double last_ms[4] = {0, 0, 0, 0};
+  std::vector<void*> allocs;
+};
+
+template <typename T>
+int dalloc(Ctx& c, T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(PIKS_E_MEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes: " +
+                                hipGetErrorString(e));
+  c.allocs.push_back(*p);
+  c.bytes += (int64_t)(n * sizeof(T));
+  HIP_TRY(hipMemsetAsync(*p, 0, n * sizeof(T), c.stream));
+  return 0;
+}
+
+template <typename T>
+int upload(Ctx& c, T** p, const std::vector<T>& v) {
+  if (int rc = dalloc(c, p, v.size())) return rc;
+  if (!v.empty()) HIP_TRY(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+  return 0;
+}
+
+int grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+void free_ctx(Ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  for (auto& e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int setup_velocity(Ctx& c) {
+  const double *x0, *w0;
+  const int n = c.prm.nc_fundamental;
+  const bool exact = c.prm.exact_quadrature != 0;
+  if (n < 1 || piks_gh_table(c.prm.half_range != 0, n, exact ? 1 : 0, &x0, &w0) != 0)
+    return fail(PIKS_E_VGRID, "Nc_fundamental xi/wi for " + std::to_string(n) + " has not been provided");
+  // src/velocityGrid.F90:166-190: nodes * sqrt(2), cx fastest, DiffFlux = 4 sum cz w
+  std::vector<double> xi(n);
+  for (int l = 0; l < n; ++l) xi[l] = x0[l] * std::sqrt(2.0);
+  const int nc8 = n * n * n;
+  c.cx.resize(nc8); c.cy.resize(nc8); c.cz.resize(nc8); c.w.resize(nc8);
+  for (int l = 0; l < n; ++l)
+    for (int m = 0; m < n; ++m)
+      for (int q = 0; q < n; ++q) {
+        const int k = q + m * n + l * n * n;
+        c.cx[k] = xi[q]; c.cy[k] = xi[m]; c.cz[k] = xi[l];
+        c.w[k] = w0[q] * w0[m] * w0[l];
+      }
+  double df = 0.0;
+  for (int l = 0; l < nc8; ++l) df = df + c.cz[l] * c.w[l];
+  c.diff_flux = df * 4;
+  c.nc_fund = n;
+  return 0;
+}
+
+// in-tile cells grouped by hyperplane a+b+c, and tiles grouped by diagonal
+int setup_tiles(Ctx& c) {
+  const int TX = c.TX, TY = c.TY, TZ = c.TZ;
+  std::vector<uint16_t> hc;
+  std::vector<int32_t> ho;
+  c.nh = TX + TY + TZ - 2;
+  for (int h = 0; h < c.nh; ++h) {
+    ho.push_back((int32_t)hc.size());
+    for (int cc = 0; cc < TZ; ++cc)
+      for (int b = 0; b < TY; ++b) {
+        const int a = h - b - cc;
+        if (a >= 0 && a < TX) hc.push_back((uint16_t)(a | (b << 5) | (cc << 10)));
+      }
+  }
+  ho.push_back((int32_t)hc.size());
+  const int NTX = (c.g.Nx + TX - 1) / TX, NTY = (c.g.Ny + TY - 1) / TY, NTZ = (c.g.Nz + TZ - 1) / TZ;
+  c.ndiag = NTX + NTY + NTZ - 2;
+  std::vector<int4> tl;
+  c.diag_off.clear();
+  for (int dg = 0; dg < c.ndiag; ++dg) {
+    c.diag_off.push_back((int)tl.size());
+    for (int K = 0; K < NTZ; ++K)
+      for (int J = 0; J < NTY; ++J) {
+        const int I = dg - J - K;
+        if (I >= 0 && I < NTX) tl.push_back(make_int4(I, J, K, 0));
+      }
+  }
+  c.diag_off.push_back((int)tl.size());
+  if (int rc = upload(c, &c.hcell, hc)) return rc;
+  if (int rc = upload(c, &c.hoff, ho)) return rc;
+  if (int rc = upload(c, &c.tiles, tl)) return rc;
+  return 0;
+}
+
+int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) {
+  if (!p || !raw || !out) return fail(PIKS_E_ARG, "null argument");
+  if (p->precision != PIKS_PREC_FP64)
+    return fail(PIKS_E_ARG, "precision mode " + std::to_string(p->precision) + " is not available");
+  if (!(p->kn > 0.0)) return fail(PIKS_E_ARG, "Kn must be positive");
+  if (p->chk_converge_step < 1) return fail(PIKS_E_ARG, "chkConvergeStep must be >= 1");
+  std::unique_ptr<Ctx, void (*)(Ctx*)> c(new Ctx(), free_ctx);
+  c->prm = *p;
+  c->exact = p->exact_order != 0;
+  c->device = p->device;
+  HIP_TRY(hipSetDevice(c->device));
+  if (int rc = setup_velocity(*c)) return rc;
+  std::string err;
+  GeomInput gi{p->nx, p->ny, p->nz, p->fluid_layer, p->wall_ext_order};
+  if (p->wall_ext_order < 1 || p->wall_ext_order > 3)
+    std::fprintf(stderr, " Error: wallExtOrder wroond shoud be [1|2|3]\n");  // src/physicalGrid.F90:947
+  if (int rc = build_geometry(gi, raw, nbytes, c->g, err)) return fail(rc, err);
+  const Geometry& g = c->g;
+  c->d = Dims{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, g.Ntotal, (int)c->cx.size()};
+  c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+
+  const size_t nf = (size_t)g.Ntotal * 8 * c->d.Nc8;
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  const size_t need = nf * sizeof(double) + (size_t)g.Ntotal * (32 * 8 + 4 * 8 + 16);
+  if (need > free_b)
+    return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
+                                std::to_string(free_b >> 20) + " MiB free");
+  int rc = 0;
+  if ((rc = dalloc(*c, &c->f, nf))) return rc;
+  if ((rc = dalloc(*c, &c->fw, (size_t)std::max(g.ncorner, 1) * 8 * c->d.Nc8))) return rc;
+  if ((rc = dalloc(*c, &c->rho, (size_t)g.Ntotal))) return rc;
+  if ((rc = dalloc(*c, &c->ux, (size_t)g.Ntotal))) return rc;
+  if ((rc = dalloc(*c, &c->uy, (size_t)g.Ntotal))) return rc;
+  if ((rc = dalloc(*c, &c->uz, (size_t)g.Ntotal))) return rc;
+  if ((rc = dalloc(*c, &c->part, (size_t)g.Ntotal * 32))) return rc;
+  const size_t nb = (size_t)g.Ny * g.Nz * 8 * c->d.Nc8;
+  if ((rc = dalloc(*c, &c->buf_e, nb))) return rc;
+  if ((rc = dalloc(*c, &c->buf_w, nb))) return rc;
+  if ((rc = dalloc(*c, &c->snap, std::max<size_t>(g.snap_cell.size(), 1) * 8 * c->d.Nc8))) return rc;
+  if ((rc = dalloc(*c, &c->plane, (size_t)g.Ny * g.Nz))) return rc;
+  if ((rc = upload(*c, &c->dcx, c->cx)) || (rc = upload(*c, &c->dcy, c->cy)) ||
+      (rc = upload(*c, &c->dcz, c->cz)) || (rc = upload(*c, &c->dw, c->w)))
+    return rc;
+  if ((rc = upload(*c, &c->lab, g.lab)) || (rc = upload(*c, &c->code, g.code)) ||
+      (rc = upload(*c, &c->wid, g.wid)) || (rc = upload(*c, &c->views, g.wall_views)) ||
+      (rc = upload(*c, &c->wall_cell, g.wall_cell)) || (rc = upload(*c, &c->wall_corner, g.wall_corner)) ||
+      (rc = upload(*c, &c->wall_label, g.wall_label)) || (rc = upload(*c, &c->e1, g.wall_e1)) ||
+      (rc = upload(*c, &c->e2, g.wall_e2)) || (rc = upload(*c, &c->uncoupled, g.uncoupled)) ||
+      (rc = upload(*c, &c->coupled, g.coupled)) || (rc = upload(*c, &c->coupled_snap, g.coupled_snap)) ||
+      (rc = upload(*c, &c->snap_cell, g.snap_cell)) || (rc = upload(*c, &c->commit_walls, g.commit_walls)) ||
+      (rc = upload(*c, &c->fluid_int, g.fluid_interior)) || (rc = upload(*c, &c->fluid_bnd, g.fluid_boundary)))
+    return rc;
+  std::vector<int32_t> all(g.fluid_interior);
+  all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
+  if ((rc = upload(*c, &c->fluid_all, all))) return rc;
+  if ((rc = setup_tiles(*c))) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *out = c.release();
+  return 0;
+}
+
+template <typename T>
+WallArgs<T> wall_args(Ctx& c, const int32_t* list, int n) {
+  WallArgs<T> a{};
+  a.f = c.f; a.fw = c.fw; a.list = list; a.n = n;
+  a.wall_cell = c.wall_cell; a.wall_label = c.wall_label; a.e1 = c.e1; a.e2 = c.e2;
+  a.wall_corner = c.wall_corner;
+  a.cx = c.dcx; a.cy = c.dcy; a.cz = c.dcz; a.w = c.dw;
+  a.accom = c.prm.accom; a.diff_flux = c.diff_flux; a.inv_diff_flux = 1.0 / c.diff_flux;
+  a.d = c.d; a.exact = c.exact;
+  a.snap_idx = c.coupled_snap; a.snap = c.snap;
+  return a;
+}
+
+int iterate(Ctx& c, int nsteps) {
+  HIP_TRY(hipSetDevice(c.device));
+  const Geometry& g = c.g;
+  const Dims d = c.d;
+  SweepArgs<double> sa{};
+  sa.f = c.f; sa.lab = c.lab; sa.code = c.code;
+  sa.rho = c.rho; sa.ux = c.ux; sa.uy = c.uy; sa.uz = c.uz;
+  sa.part = c.exact ? nullptr : c.part;
+  sa.wid = c.wid; sa.views = c.views; sa.wall_corner = c.wall_corner; sa.fw = c.fw;
+  sa.cx = c.dcx; sa.cy = c.dcy; sa.cz = c.dcz; sa.w = c.dw;
+  sa.mu = c.mu;
+  sa.c0 = 1.5 / g.ds; sa.c1 = 2.0 / g.ds; sa.c2 = -0.5 / g.ds; sa.d1 = 1.0 / g.ds;
+  sa.d = d; sa.TX = c.TX; sa.TY = c.TY; sa.TZ = c.TZ;
+  sa.hcell = c.hcell; sa.hoff = c.hoff; sa.nh = c.nh;
+  const int wpb = 4;  // waves per block for one-wave-per-item kernels
+  HIP_TRY(hipEventRecord(c.ev[4], c.stream));
+  for (int step = 0; step < nsteps; ++step) {
+    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[0], c.stream));
+    for (int dg = 0; dg < c.ndiag; ++dg) {
+      const int nt = c.diag_off[dg + 1] - c.diag_off[dg];
+      sa.tiles = c.tiles + c.diag_off[dg];
+      hipLaunchKernelGGL(k_sweep<double>, dim3(nt * 8), dim3(256), 0, c.stream, sa);
+    }
+    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[1], c.stream));
+    if (!g.commit_walls.empty())
+      hipLaunchKernelGGL(k_commit<double>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb)),
+                         dim3(wpb * kWave), 0, c.stream, c.f, c.fw, c.commit_walls,
+                         (int)g.commit_walls.size(), c.wall_cell, c.wall_corner, c.views, d);
+    hipLaunchKernelGGL(k_halo<double>, dim3(grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256)), dim3(256),
+                       0, c.stream, c.f, c.buf_e, c.buf_w, d);
+    if (!g.snap_cell.empty())
+      hipLaunchKernelGGL(k_snapshot<double>, dim3(grid_for((int64_t)g.snap_cell.size() * 8 * d.Nc8, 256)),
+                         dim3(256), 0, c.stream, (const double*)c.f, c.snap, c.snap_cell,
+                         (int)g.snap_cell.size(), d);
+    if (!g.uncoupled.empty()) {
+      WallArgs<double> wa = wall_args<double>(c, c.uncoupled, (int)g.uncoupled.size());
+      hipLaunchKernelGGL(k_walls<double>, dim3(grid_for((int64_t)g.uncoupled.size(), wpb)),
+                         dim3(wpb * kWave), 0, c.stream, wa);
+    }
+    if (!g.coupled.empty()) {
+      WallArgs<double> wa = wall_args<double>(c, c.coupled, (int)g.coupled.size());
+      hipLaunchKernelGGL(k_walls_serial<double>, dim3(1), dim3(kWave), 0, c.stream, wa);
+    }
+    hipLaunchKernelGGL(k_inout<double>, dim3(grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256)), dim3(256),
+                       0, c.stream, c.f, (const double*)c.dw, c.prm.press_drop, d);
+    hipLaunchKernelGGL(k_sym_y<double>, dim3(grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256)), dim3(256), 0,
+                       c.stream, c.f, d);
+    hipLaunchKernelGGL(k_sym_z<double>, dim3(grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256)), dim3(256), 0,
+                       c.stream, c.f, d);
+    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[2], c.stream));
+    if (c.exact) {
+      const int64_t n = (int64_t)g.fluid_interior.size() + (int64_t)g.fluid_boundary.size();
+      hipLaunchKernelGGL(k_moments_serial<double>, dim3(grid_for(n, 64)), dim3(64), 0, c.stream,
+                         (const double*)c.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, c.rho, c.ux,
+                         c.uy, c.uz);
+    } else {
+      const int64_t ni = (int64_t)g.fluid_interior.size(), nbd = (int64_t)g.fluid_boundary.size();
+      if (ni > 0)
+        hipLaunchKernelGGL(k_moments_partials, dim3(grid_for(ni, 256)), dim3(256), 0, c.stream,
+                           (const double*)c.part, c.fluid_int, ni, c.rho, c.ux, c.uy, c.uz);
+      if (nbd > 0)
+        hipLaunchKernelGGL(k_moments_direct<double>, dim3((unsigned)((nbd + wpb - 1) / wpb)),
+                           dim3(wpb * kWave), 0, c.stream, (const double*)c.f, c.fluid_bnd, nbd,
+                           c.dcx, c.dcy, c.dcz, d, c.rho, c.ux, c.uy, c.uz);
+    }
+    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[3], c.stream));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  if (nsteps > 0) {
+    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+    HIP_TRY(hipEventElapsedTime(&t03, c.ev[4], c.ev[3]));
+    HIP_TRY(hipEventElapsedTime(&t01, c.ev[0], c.ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t12, c.ev[1], c.ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t23, c.ev[2], c.ev[3]));
+    c.last_ms[0] = t01; c.last_ms[1] = t12; c.last_ms[2] = t23; c.last_ms[3] = t03;
+  }
+  return 0;
+}
+
+// src/output.F90:80-142 on the host, in the reference's summation order
+int mass_flow(Ctx& c, double* m) {
+  const Geometry& g = c.g;
+  const Dims d = c.d;
+  HIP_TRY(hipSetDevice(c.device));
+  hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)g.Ny * g.Nz, 256)), dim3(256), 0, c.stream,
+                     (const double*)c.ux, c.plane, d, 2);
+  std::vector<double> u((size_t)g.Ny * g.Nz);
+  HIP_TRY(hipMemcpyAsync(u.data(), c.plane, u.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  auto U = [&](int j, int k) { return u[(size_t)(j - 1) + (size_t)(k - 1) * g.Ny]; };
+  const double ds = g.ds;
+  const int Ny = g.Ny, Nz = g.Nz;
+  double inner = 0.0, suth = 0.0, noth = 0.0, frnt = 0.0, back = 0.0;
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny - 1; ++j) inner = inner + U(j, k) * ds * ds;
+  for (int k = 2; k <= Nz - 1; ++k) suth = suth + U(1, k) * ds * ds * 0.5;
+  for (int k = 2; k <= Nz - 1; ++k) noth = noth + U(Ny, k) * ds * ds * 0.5;
+  for (int j = 2; j <= Ny - 1; ++j) back = back + U(j, 1) * ds * ds * 0.5;
+  for (int j = 2; j <= Ny - 1; ++j) frnt = frnt + U(j, Nz) * ds * ds * 0.5;
+  const double sb = U(1, 1) * ds * ds * 0.25, sf = U(1, Nz) * ds * ds * 0.25;
+  const double nb = U(Ny, 1) * ds * ds * 0.25, nf = U(Ny, Nz) * ds * ds * 0.25;
+  const double rl = 1.0 / c.prm.ref_l;
+  *m = (inner + suth + noth + frnt + back + sb + sf + nb + nf) * std::sqrt(1.0 / 2.0) /
+       c.prm.press_drop / (rl * rl);
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+int piks_create(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes, void** ctx_out) {
+  g_err.clear();
+  if (ctx_out) *ctx_out = nullptr;
+  Ctx* c = nullptr;
+  int rc = create(p, raw_image, image_bytes, &c);
+  if (rc == 0) *ctx_out = c;
+  return rc;
+}
+
+int piks_iterate(void* ctx, int32_t nsteps) {
+  if (!ctx || nsteps < 0) return fail(PIKS_E_ARG, "bad context or step count");
+  return iterate(*static_cast<Ctx*>(ctx), nsteps);
+}
+
+int piks_mass_flow(void* ctx, double* m) {
+  if (!ctx || !m) return fail(PIKS_E_ARG, "null argument");
+  return mass_flow(*static_cast<Ctx*>(ctx), m);
+}
+
+int piks_chk_converge(void* ctx, double out[3]) {
+  if (!ctx || !out) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  double m = 0.0;
+  if (int rc = mass_flow(c, &m)) return rc;
+  // src/output.F90:151-155
+  const double error = std::fabs(1.0 - m / c.mass) / (double)c.prm.chk_converge_step;
+  c.mass = m;
+  out[0] = m;
+  out[1] = m * c.prm.kn * std::sqrt(4.0 / (std::atan(1.0) * 4.0));
+  out[2] = error;
+  return 0;
+}
+
+int piks_get_macro(void* ctx, double* rho, double* ux, double* uy, double* uz) {
+  if (!ctx) return fail(PIKS_E_ARG, "null context");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  HIP_TRY(hipSetDevice(c.device));
+  double* dst[4] = {rho, ux, uy, uz};
+  double* src[4] = {c.rho, c.ux, c.uy, c.uz};
+  for (int q = 0; q < 4; ++q)
+    if (dst[q])
+      HIP_TRY(hipMemcpyAsync(dst[q], src[q], (size_t)c.g.Ntotal * sizeof(double), hipMemcpyDeviceToHost,
+                             c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int piks_get_f(void* ctx, double* f) {
+  if (!ctx || !f) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  HIP_TRY(hipSetDevice(c.device));
+  const size_t nc8 = c.d.Nc8, nt = (size_t)c.g.Ntotal;
+  std::vector<double> h(nt * 8 * nc8);
+  HIP_TRY(hipMemcpyAsync(h.data(), c.f, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  // device [o][cell][l] -> Fortran f(cell, l, o)
+  for (size_t o = 0; o < 8; ++o)
+    for (size_t cell = 0; cell < nt; ++cell)
+      for (size_t l = 0; l < nc8; ++l) f[(o * nc8 + l) * nt + cell] = h[(o * nt + cell) * nc8 + l];
+  return 0;
+}
+
+int piks_get_fw(void* ctx, double* fw) {
+  if (!ctx || !fw) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  HIP_TRY(hipSetDevice(c.device));
+  const size_t nc8 = c.d.Nc8, ncor = (size_t)c.g.ncorner;
+  if (ncor == 0) return 0;
+  std::vector<double> h(ncor * 8 * nc8);
+  HIP_TRY(hipMemcpyAsync(h.data(), c.fw, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  for (size_t o = 0; o < 8; ++o)
+    for (size_t q = 0; q < ncor; ++q)
+      for (size_t l = 0; l < nc8; ++l) fw[(o * nc8 + l) * ncor + q] = h[(q * 8 + o) * nc8 + l];
+  return 0;
+}
+
+int piks_get_labels(void* ctx, int32_t* labels) {
+  if (!ctx || !labels) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  for (size_t q = 0; q < c.g.lab.size(); ++q) labels[q] = c.g.lab[q];
+  return 0;
+}
+
+int piks_get_info(void* ctx, piks_info* info) {
+  if (!ctx || !info) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  const Geometry& g = c.g;
+  std::memset(info, 0, sizeof(*info));
+  info->nx = g.Nx; info->ny = g.Ny; info->nz = g.Nz;
+  info->nxt = g.Nxt; info->nyt = g.Nyt; info->nzt = g.Nzt;
+  info->ntotal = g.Ntotal;
+  info->nc8 = c.d.Nc8;
+  info->nwall = (int32_t)g.wall_cell.size();
+  info->ncorner = g.ncorner;
+  info->nfluid = g.nfluid;
+  info->updates_per_iter = g.nfluid * 8 * (int64_t)c.d.Nc8;
+  info->ds = g.ds; info->mu = c.mu; info->diff_flux = c.diff_flux;
+  info->mass = c.mass;
+  info->device_bytes = c.bytes;
+  info->coupled_walls = (int32_t)g.coupled.size();
+  info->precision = c.prm.precision;
+  return 0;
+}
+
+int piks_last_timing(void* ctx, double ms[4]) {
+  if (!ctx || !ms) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  for (int q = 0; q < 4; ++q) ms[q] = c.last_ms[q];
+  return 0;
+}
+
+int piks_destroy(void* ctx) {
+  free_ctx(static_cast<Ctx*>(ctx));
+  return 0;
+}
+
+const char* piks_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"
