@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput of the DVM source iteration on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): a 200^3 image
+of randomly placed overlapping solid spheres (centres uniform in the box,
+radii uniform in [4, 10] voxels, added until the porosity is 0.40, PCG64 seed
+2024), 4 fluid layers at inlet and outlet (208 x 200 x 200 domain), 8^3
+half-range Gauss-Hermite velocities, Kn = 1, pressure drop 1 along x, diffuse
+walls (accom = 1), second-order wall extrapolation.
+
+One step = one source iteration (eight octant sweeps, wall and inlet/outlet
+conditions, symmetry planes, moments) over the whole image with f resident
+in HBM.  value = cell*velocity updates per second = Nfluid * 512 * steps /
+wall time of the timed steps.  roofline: the sweep kernels' algorithmic HBM
+bytes (16 B per update in FP64: read the previous f, write the new one)
+divided by their device time from HIP events, against 8 TB/s.
+cpu_baseline: the reference program itself (built from its own sources into
+oracle/_ref), 2 iterations of the same workload on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--precision fp64]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "cell·velocity updates/s; batched solves/s + p99 latency at 1e-8 perm error"
+HBM_PEAK_GBS = 8000.0
+SEED = 2024
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload():
+    from piks3d_amd import images
+    from piks3d_amd.params import Params
+    img = images.sphere_pack(200, 0.40, 4.0, 10.0, seed=SEED)
+    p = Params(image_file_name="spheres200.raw", nx=200, ny=200, nz=200, fluid_layer=4,
+               wall_ext_order=2, ref_l=1.0, nc_fundamental=4, half_range=True,
+               chk_converge_step=10, max_step=2, eps=0.0, kn=1.0, press_drop=1.0, accom=1.0)
+    return p, img
+
+
+def cpu_baseline(p, img, nfluid, nc, threads, iters=2):
+    """Time the reference program (oracle/_ref/piks3d) on `iters` iterations."""
+    from piks3d_amd.params import format_para
+    from piks3d_amd import images
+    exe = os.path.join(HERE, "oracle", "_ref", "piks3d")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory(prefix="piks_ref_") as d:
+        images.write_raw(os.path.join(d, p.image_file_name), img)
+        q = dict(p.__dict__)
+        from piks3d_amd.params import Params
+        pr = Params(**q)
+        pr.max_step = iters
+        pr.chk_converge_step = 1000
+        pr.save_last = False
+        pr.save_step = 1 << 30
+        with open(os.path.join(d, "para.in"), "w") as fh:
+            fh.write(format_para(pr))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        log("cpu baseline: reference program, %d threads, %d iterations" % (threads, iters))
+        r = subprocess.run([exe], cwd=d, env=env, capture_output=True, text=True, timeout=900)
+        m = re.search(r"Walltime=\s*([0-9.Ee+-]+)", r.stdout)
+        if r.returncode != 0 or not m:
+            log("reference run failed:", r.stdout[-500:], r.stderr[-500:])
+            return None
+        wall = float(m.group(1))
+    return {"value": nfluid * nc * iters / wall, "unit": "updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": "%d iterations of the same 208x200x200 workload, reference PIKS3D "
+                      "(flang -O2 -fopenmp, 1 MPI rank), loop wall time %.3g s" % (iters, wall)}
+
+
+def traffic_from_profiles(workload_name):
+    path = os.path.join(HERE, "profiles", "traffic_%s.json" % workload_name)
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only; piks_iterate synchronizes its stream itself
+        sync = lambda: None  # noqa: E731
+
+    from piks3d_amd import Solver
+    log("generating the 200^3 sphere pack")
+    p, img = workload()
+    t = time.perf_counter()
+    s = Solver(p, img, precision=args.precision)
+    info = s.info
+    log("set-up %.1f s: Nfluid %d, walls %d, corners %d, device memory %.1f GB" % (
+        time.perf_counter() - t, info.nfluid, info.nwall, info.ncorner, info.device_bytes / 1e9))
+    nc = info.nc8 * 8
+    if args.warmup:
+        s.iterate(args.warmup)
+    log("warm-up done (%d steps)" % args.warmup)
+    sync()
+    t0 = time.perf_counter()
+    s.iterate(args.steps)
+    sync()
+    t1 = time.perf_counter()
+    ms = s.timing()
+    elapsed = t1 - t0
+    updates = info.nfluid * nc * args.steps
+    value = updates / elapsed
+    bytes_per_update = 16 if args.precision == "fp64" else 8
+    sweep_s = ms[0] / 1e3
+    achieved = updates * bytes_per_update / sweep_s / 1e9
+    m, k, err = s.chk_converge()
+    log("timed %d steps in %.3f s: %.4g updates/s; phases (ms/step): sweep %.3f, walls+bc %.3f, "
+        "moments %.3f; K after %d steps = %.6e" % (
+            args.steps, elapsed, value, ms[0] / args.steps, ms[1] / args.steps, ms[2] / args.steps,
+            args.warmup + args.steps, k))
+    s.close()
+
+    wl = "spheres200_phi040_kn1_v8"
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(wl),
+            "kernel": "k_sweep", "bytes_per_update": bytes_per_update,
+            "sweep_ms_per_step": round(ms[0] / args.steps, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, img, info.nfluid, nc, args.cpu_threads)
+    out = {
+        "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": wl, "image": "200^3 overlapping spheres r~U[4,10], porosity 0.40, seed %d" % SEED,
+                   "domain": [info.nx, info.ny, info.nz], "fluid_cells": info.nfluid,
+                   "velocities": nc, "kn": p.kn, "precision": args.precision},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "phase_ms_per_step": [round(x / args.steps, 4) for x in ms[:3]],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -120,10 +120,10 @@ int piks_get_labels(void* ctx, int32_t* labels);
 
 int piks_get_info(void* ctx, piks_info* info);
 
-/* Device time of piks_iterate's last step, split by phase (milliseconds):
- * [0] sweeps, [1] switch commit + exchange + wall BC + inlet/outlet + symmetry,
- * [2] moments; [3] the whole call (all steps).  HIP events on the context's
- * stream. */
+/* Device time of the last piks_iterate call, summed over its steps (the first
+ * 4096) and split by phase (milliseconds): [0] sweeps, [1] switch commit +
+ * exchange + wall BC + inlet/outlet + symmetry, [2] moments, [3] first step's
+ * start to last step's end.  HIP events on the context's stream. */
 int piks_last_timing(void* ctx, double ms[4]);
 
 int piks_destroy(void* ctx);

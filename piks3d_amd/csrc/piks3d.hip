@@ -49,6 +49,7 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kWave = 64;
+constexpr int kMaxTimedSteps = 4096;  // steps of one piks_iterate call timed by events
 
 // ------------------------------------------------------------------ helpers
 
@@ -604,8 +605,8 @@ struct Ctx {
   uint16_t* hcell = nullptr;
   int32_t* hoff = nullptr;
   int nh = 0;
-  // timing
-  hipEvent_t ev[5] = {};
+  // timing: four events per step of the last piks_iterate call
+  std::vector<hipEvent_t> ev;
   double last_ms[4] = {0, 0, 0, 0};
   std::vector<void*> allocs;
 };
@@ -640,8 +641,7 @@ void free_ctx(Ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) hipFree(p);
-  for (auto& e : c->ev)
-    if (e) hipEventDestroy(e);
+  for (auto& e : c->ev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -726,7 +726,6 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   c->d = Dims{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, g.Ntotal, (int)c->cx.size()};
   c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
 
   const size_t nf = (size_t)g.Ntotal * 8 * c->d.Nc8;
   size_t free_b = 0, total_b = 0;
@@ -797,15 +796,21 @@ int iterate(Ctx& c, int nsteps) {
   sa.d = d; sa.TX = c.TX; sa.TY = c.TY; sa.TZ = c.TZ;
   sa.hcell = c.hcell; sa.hoff = c.hoff; sa.nh = c.nh;
   const int wpb = 4;  // waves per block for one-wave-per-item kernels
-  HIP_TRY(hipEventRecord(c.ev[4], c.stream));
+  const int timed = std::min(nsteps, kMaxTimedSteps);
+  while ((int)c.ev.size() < 4 * timed) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    c.ev.push_back(e);
+  }
   for (int step = 0; step < nsteps; ++step) {
-    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[0], c.stream));
+    const bool tm = step < timed;
+    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 0], c.stream));
     for (int dg = 0; dg < c.ndiag; ++dg) {
       const int nt = c.diag_off[dg + 1] - c.diag_off[dg];
       sa.tiles = c.tiles + c.diag_off[dg];
       hipLaunchKernelGGL(k_sweep<double>, dim3(nt * 8), dim3(256), 0, c.stream, sa);
     }
-    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[1], c.stream));
+    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
       hipLaunchKernelGGL(k_commit<double>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb)),
                          dim3(wpb * kWave), 0, c.stream, c.f, c.fw, c.commit_walls,
@@ -831,7 +836,7 @@ int iterate(Ctx& c, int nsteps) {
                        c.stream, c.f, d);
     hipLaunchKernelGGL(k_sym_z<double>, dim3(grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256)), dim3(256), 0,
                        c.stream, c.f, d);
-    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[2], c.stream));
+    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 2], c.stream));
     if (c.exact) {
       const int64_t n = (int64_t)g.fluid_interior.size() + (int64_t)g.fluid_boundary.size();
       hipLaunchKernelGGL(k_moments_serial<double>, dim3(grid_for(n, 64)), dim3(64), 0, c.stream,
@@ -847,17 +852,21 @@ int iterate(Ctx& c, int nsteps) {
                            dim3(wpb * kWave), 0, c.stream, (const double*)c.f, c.fluid_bnd, nbd,
                            c.dcx, c.dcy, c.dcz, d, c.rho, c.ux, c.uy, c.uz);
     }
-    if (step == nsteps - 1) HIP_TRY(hipEventRecord(c.ev[3], c.stream));
+    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 3], c.stream));
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c.stream));
-  if (nsteps > 0) {
-    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
-    HIP_TRY(hipEventElapsedTime(&t03, c.ev[4], c.ev[3]));
-    HIP_TRY(hipEventElapsedTime(&t01, c.ev[0], c.ev[1]));
-    HIP_TRY(hipEventElapsedTime(&t12, c.ev[1], c.ev[2]));
-    HIP_TRY(hipEventElapsedTime(&t23, c.ev[2], c.ev[3]));
-    c.last_ms[0] = t01; c.last_ms[1] = t12; c.last_ms[2] = t23; c.last_ms[3] = t03;
+  if (timed > 0) {
+    double acc[3] = {0, 0, 0};
+    for (int st = 0; st < timed; ++st)
+      for (int q = 0; q < 3; ++q) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, c.ev[4 * st + q], c.ev[4 * st + q + 1]));
+        acc[q] += t;
+      }
+    float all = 0;
+    HIP_TRY(hipEventElapsedTime(&all, c.ev[0], c.ev[4 * (timed - 1) + 3]));
+    c.last_ms[0] = acc[0]; c.last_ms[1] = acc[1]; c.last_ms[2] = acc[2]; c.last_ms[3] = all;
   }
   return 0;
 }
