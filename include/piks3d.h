@@ -120,6 +120,13 @@ int piks_get_labels(void* ctx, int32_t* labels);
 
 int piks_get_info(void* ctx, piks_info* info);
 
+/* Host-only image preprocessing (no device needed): the labelled image of
+ * src/physicalGrid.F90:251-671 (Ntotal int32 into labels, may be NULL) and
+ * its counts in info (nx..ntotal, nc8 = 0, nwall, ncorner, nfluid,
+ * coupled_walls).  Same error behaviour as piks_create. */
+int piks_preprocess(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
+                    int32_t* labels, piks_info* info);
+
 /* Device time of the last piks_iterate call, summed over its steps (the first
  * 4096) and split by phase (milliseconds): [0] sweeps, [1] switch commit +
  * exchange + wall BC + inlet/outlet + symmetry, [2] moments, [3] first step's

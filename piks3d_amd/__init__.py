@@ -69,6 +69,8 @@ _API = {
     "piks_get_fw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "piks_get_labels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "piks_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PiksInfo)]),
+    "piks_preprocess": (ctypes.c_int, [ctypes.POINTER(PiksParams), ctypes.c_void_p, ctypes.c_int64,
+                                       ctypes.c_void_p, ctypes.POINTER(PiksInfo)]),
     "piks_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "piks_last_error": (ctypes.c_char_p, []),
@@ -120,6 +122,18 @@ def to_cparams(p: Params, precision=None, exact_order=False, device=None) -> Pik
     cp.exact_quadrature = int(bool(p.exact_quadrature))
     cp.exact_order = int(bool(exact_order))
     return cp
+
+
+def preprocess(params: Params, image):
+    """Labels (reference cell order) and counts of an image, on the host only."""
+    raw = np.ascontiguousarray(np.asarray(image, dtype=np.uint8).ravel())
+    cp = to_cparams(params)
+    info = PiksInfo()
+    _check(lib().piks_preprocess(ctypes.byref(cp), raw.ctypes.data, raw.size, None, ctypes.byref(info)))
+    labels = np.empty(info.ntotal, dtype=np.int32)
+    _check(lib().piks_preprocess(ctypes.byref(cp), raw.ctypes.data, raw.size, labels.ctypes.data,
+                                 ctypes.byref(info)))
+    return labels, info
 
 
 class Solver:

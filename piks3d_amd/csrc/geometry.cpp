@@ -1,7 +1,10 @@
 // geometry.cpp -- image -> device layouts (see geometry.h).
 #include "geometry.h"
 
+#include <algorithm>
+#include <array>
 #include <cstring>
+#include <unordered_map>
 
 #include "../../include/piks3d.h"
 #include "wallprog.h"
@@ -276,39 +279,105 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
         g.code[(size_t)c] = code;
       }
 
-  // walls whose extrapolation reads another wall: the reference's serial
-  // wall loop gives them earlier walls' new values and later walls' old ones
-  std::vector<int32_t> snap_of_cell;
+  // walls whose extrapolation reads another wall (see geometry.h)
+  std::vector<int> level(nwall, 0);
+  std::vector<std::array<int32_t, 6>> slots_of(nwall);
+  std::unordered_map<int32_t, int32_t> snap_of_cell;
+  std::vector<char> is_coupled(nwall, 0);
+  int maxlevel = 0;
   for (size_t w = 0; w < nwall; ++w) {
     WallProg P = wall_prog(g.wall_label[w]);
     int64_t c = g.wall_cell[w];
-    bool coupled = false;
-    int32_t slots[6] = {-1, -1, -1, -1, -1, -1};
+    std::array<int32_t, 6> slots = {-1, -1, -1, -1, -1, -1};
+    int lev = 0;
     for (int a = 0; a < P.nface; ++a)
       for (int d = 1; d <= 2; ++d) {
         int64_t n = c + (int64_t)d * P.face[a].dir * stride[(int)P.face[a].axis];
         int wn = g.wid[(size_t)n];
         if (wn < 0) continue;
-        coupled = true;
+        is_coupled[w] = 1;
         if ((size_t)wn > w) {  // later in the wall order: needs its old value
-          int idx = -1;
-          for (size_t q = 0; q < g.snap_cell.size(); ++q)
-            if (g.snap_cell[q] == (int32_t)n) idx = (int)q;
-          if (idx < 0) {
-            idx = (int)g.snap_cell.size();
+          auto it = snap_of_cell.find((int32_t)n);
+          int32_t idx;
+          if (it == snap_of_cell.end()) {
+            idx = (int32_t)g.snap_cell.size();
+            snap_of_cell[(int32_t)n] = idx;
             g.snap_cell.push_back((int32_t)n);
+          } else {
+            idx = it->second;
           }
           slots[a * 2 + (d - 1)] = idx;
+        } else {
+          lev = std::max(lev, level[wn]);  // earlier wall: must run before
         }
       }
-    if (coupled) {
-      g.coupled.push_back((int32_t)w);
-      for (int q = 0; q < 6; ++q) g.coupled_snap.push_back(slots[q]);
+    if (is_coupled[w]) {
+      level[w] = lev + 1;
+      maxlevel = std::max(maxlevel, level[w]);
+      slots_of[w] = slots;
     } else {
       g.uncoupled.push_back((int32_t)w);
     }
   }
+  g.coupled_level_off.push_back(0);
+  for (int L = 1; L <= maxlevel; ++L) {
+    for (size_t w = 0; w < nwall; ++w)
+      if (is_coupled[w] && level[w] == L) {
+        g.coupled.push_back((int32_t)w);
+        for (int q = 0; q < 6; ++q) g.coupled_snap.push_back(slots_of[w][q]);
+      }
+    g.coupled_level_off.push_back((int32_t)g.coupled.size());
+  }
+  build_sweep_schedule(g, g.TX, g.TY, g.TZ);
   return PIKS_OK;
+}
+
+void build_sweep_schedule(Geometry& g, int tx, int ty, int tz) {
+  g.TX = tx; g.TY = ty; g.TZ = tz;
+  g.NTX = (g.Nx + tx - 1) / tx;
+  g.NTY = (g.Ny + ty - 1) / ty;
+  g.NTZ = (g.Nz + tz - 1) / tz;
+  g.nh = tx + ty + tz - 2;
+  const int64_t NT = (int64_t)g.NTX * g.NTY * g.NTZ;
+  const int nh1 = g.nh + 1;
+  g.sweep_off.assign((size_t)(8 * NT * nh1), 0);
+  std::vector<int64_t> key;     // bucket of each swept cell, per octant
+  std::vector<int32_t> cells;
+  g.sweep_cells.clear();
+  for (int o = 0; o < 8; ++o) {
+    const int sx = osx(o), sy = osy(o), sz = osz(o);
+    std::vector<int32_t> count((size_t)(NT * g.nh), 0);
+    key.clear();
+    cells.clear();
+    for (int k = 1; k <= g.Nz; ++k)
+      for (int j = 1; j <= g.Ny; ++j)
+        for (int i = 1; i <= g.Nx; ++i) {
+          int64_t c = g.cell(i, j, k);
+          if (g.lab[(size_t)c] != FLUID) continue;
+          const int oi = sx > 0 ? i - 1 : g.Nx - i;
+          const int oj = sy > 0 ? j - 1 : g.Ny - j;
+          const int ok = sz > 0 ? k - 1 : g.Nz - k;
+          if (oi < 1 || oj < 1 || ok < 1) continue;  // the octant's inflow layer is not swept
+          const int64_t t = ((int64_t)(oi / tx) * g.NTY + oj / ty) * g.NTZ + ok / tz;
+          const int h = oi % tx + oj % ty + ok % tz;
+          const int64_t b = t * g.nh + h;
+          key.push_back(b);
+          cells.push_back((int32_t)c);
+          ++count[(size_t)b];
+        }
+    // counting sort by (tile, hyperplane); within a bucket the raster order is kept
+    std::vector<int64_t> start((size_t)(NT * g.nh) + 1, 0);
+    for (size_t b = 0; b < count.size(); ++b) start[b + 1] = start[b] + count[b];
+    const int64_t base = (int64_t)g.sweep_cells.size();
+    g.sweep_cells.resize((size_t)(base + (int64_t)cells.size()));
+    std::vector<int64_t> pos(start.begin(), start.end() - 1);
+    for (size_t q = 0; q < cells.size(); ++q) g.sweep_cells[(size_t)(base + pos[(size_t)key[q]]++)] = cells[q];
+    for (int64_t t = 0; t < NT; ++t) {
+      int32_t* off = &g.sweep_off[(size_t)(((int64_t)o * NT + t) * nh1)];
+      for (int h = 0; h < g.nh; ++h) off[h] = (int32_t)(base + start[(size_t)(t * g.nh + h)]);
+      off[g.nh] = (int32_t)(base + start[(size_t)((t + 1) * g.nh)]);
+    }
+  }
 }
 
 }  // namespace piks

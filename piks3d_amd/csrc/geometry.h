@@ -38,9 +38,13 @@ struct Geometry {
   std::vector<uint64_t> wall_views; // 8 octants x (y code, z code) nibbles
   std::vector<int32_t> wid;         // Ntotal: wall index or -1
   int ncorner = 0;
-  // walls whose extrapolation reads another wall (order-dependent in the
-  // reference), in vecWall order, with per (face, distance) snapshot slots
-  std::vector<int32_t> coupled;           // wall indices
+  // Walls whose extrapolation reads another wall: the reference's serial wall
+  // loop gives them the new values of earlier walls and the old values of
+  // later ones.  They run after the others in dependency levels (a wall's
+  // level is one more than the highest level of the earlier walls it reads);
+  // reads of later walls come from a snapshot taken before the wall pass.
+  std::vector<int32_t> coupled;           // wall indices, sorted by level
+  std::vector<int32_t> coupled_level_off; // offsets of each level in `coupled`
   std::vector<int32_t> coupled_snap;      // 6 per coupled wall: snapshot index or -1
   std::vector<int32_t> snap_cell;         // cells to snapshot before the wall pass
   std::vector<int32_t> uncoupled;         // wall indices processed in parallel
@@ -50,6 +54,15 @@ struct Geometry {
   std::vector<int32_t> commit_walls;      // walls with a switch in some octant
   int64_t nfluid = 0;
 
+  // Sweep schedule.  Each octant tiles the interior in its own oriented frame
+  // (oriented x = i-1 for cx > 0, Nx-i for cx < 0; likewise y, z) with tiles
+  // of TX x TY x TZ cells.  sweep_cells holds, per octant and tile, the cells
+  // the octant updates, ordered by in-tile hyperplane a+b+c;
+  // sweep_off[((o*NT + t)*(nh+1)) + h] is where hyperplane h of tile t starts.
+  int TX = 8, TY = 8, TZ = 8, NTX = 0, NTY = 0, NTZ = 0, nh = 0;
+  std::vector<int32_t> sweep_cells;
+  std::vector<int32_t> sweep_off;
+
   int64_t cell(int i, int j, int k) const {
     return (int64_t)(i + 1) + (int64_t)(j + 1) * Nxt + (int64_t)(k + 1) * Nxyt;
   }
@@ -58,5 +71,8 @@ struct Geometry {
 // Returns 0 on success; on failure returns a PIKS_E* code and fills err.
 int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geometry& g,
                    std::string& err);
+
+// Fill the sweep schedule for tiles of tx x ty x tz cells (each <= 32).
+void build_sweep_schedule(Geometry& g, int tx, int ty, int tz);
 
 }  // namespace piks

@@ -76,29 +76,31 @@ struct Dims {
 template <typename T>
 struct SweepArgs {
   T* f;
-  const uint8_t* lab;
   const uint32_t* code;
   const double* rho;
   const double* ux;
   const double* uy;
   const double* uz;
-  double* part;            // [cell][octant][4] partial moments; null = skip
+  double* part;            // [cell][octant][chunk][4] partial moments; null = skip
   const int32_t* wid;
   const uint64_t* views;
   const int32_t* wall_corner;
   const T* fw;             // [corner][octant][velocity]
+  const uint8_t* lab;
   const double* cx;
   const double* cy;
   const double* cz;
   const double* w;
   double mu, c0, c1, c2, d1;  // 1.5/ds, 2/ds, -0.5/ds, 1/ds
   Dims d;
-  int TX, TY, TZ;
-  const int4* tiles;       // tiles of this diagonal (oriented tile coords)
-  const uint16_t* hcell;   // in-tile cells grouped by hyperplane
-  const int32_t* hoff;     // nh + 1 offsets
-  int nh;
+  const int4* tiles;       // oriented tile coordinates of this diagonal
+  const int32_t* cells;    // swept cells per (octant, tile), by in-tile hyperplane
+  const int32_t* off;      // hyperplane offsets into `cells`
+  int NTY, NTZ, nh, nchunk;
+  int64_t NT;
 };
+
+constexpr int kBatch = 4;  // cells whose loads a wave has in flight together
 
 // Value a cell reading neighbour n along `axis` (1 = y, 2 = z) sees in octant
 // o, when the reference would have switched that wall slot by then; null if
@@ -114,87 +116,116 @@ __device__ __forceinline__ const T* switched_source(const SweepArgs<T>& a, int64
   return a.fw + ((size_t)a.wall_corner[w] * 8 + o) * a.d.Nc8;
 }
 
-// One update of f(cell, l, o), src/solver.F90:206-214 and analogues, with the
-// same operations in the same order (built with -ffp-contract=off): signs are
-// applied as exact negations, zero-weight second-order terms are skipped.
+// Sum of four per-lane values over the wave in 7 exchanges: lane 0 ends with
+// the sum of a, lane 16 of b, lane 32 of c, lane 48 of d.
+__device__ __forceinline__ double wave_sum4(double a, double b, double c, double d, int lane) {
+  const bool up = lane & 32;
+  const double r0 = __shfl_xor(up ? a : c, 32, kWave);
+  const double r1 = __shfl_xor(up ? b : d, 32, kWave);
+  const double x = (up ? c : a) + r0;
+  const double y = (up ? d : b) + r1;
+  const bool b4 = lane & 16;
+  double z = (b4 ? y : x) + __shfl_xor(b4 ? x : y, 16, kWave);
+  z += __shfl_xor(z, 8, kWave);
+  z += __shfl_xor(z, 4, kWave);
+  z += __shfl_xor(z, 2, kWave);
+  z += __shfl_xor(z, 1, kWave);
+  return z;
+}
+
+// One octant's sweep over one tile for one chunk of 64 velocities.  Cells of
+// in-tile hyperplane h depend only on hyperplanes h-1, h-2 and on upstream
+// tiles (earlier launches), so a barrier per hyperplane orders the tile.
+// Each update is src/solver.F90:206-214 (and analogues) with the same
+// operations in the same order (contraction off): signs enter as exact
+// negations, and a first-order axis contributes (c*0)*f like the reference.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
-  const int o = blockIdx.x & 7;
-  const int4 tile = a.tiles[blockIdx.x >> 3];
-  const int sx = osx(o), sy = osy(o), sz = osz(o);
+  const int nch = a.nchunk;
+  const int ch = blockIdx.x % nch;
+  const int o = (blockIdx.x / nch) & 7;
+  const int4 tile = a.tiles[blockIdx.x / (nch * 8)];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int nwaves = blockDim.x / kWave;
-  const int64_t dx = sx, dy = (int64_t)sy * a.d.Nxt, dz = (int64_t)sz * a.d.Nxyt;
   const int Nc8 = a.d.Nc8;
-  T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8;
+  const int l = ch * kWave + lane;
+  const bool lact = l < Nc8;
+  const int ll = lact ? l : 0;
+  const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
+  const int sx = osx(o), sy = osy(o), sz = osz(o);
+  const int64_t dx = sx, dy = (int64_t)sy * a.d.Nxt, dz = (int64_t)sz * a.d.Nxyt;
+  T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8 + ll;
+  const int64_t tid = ((int64_t)tile.x * a.NTY + tile.y) * a.NTZ + tile.z;
+  const int32_t* off = a.off + ((int64_t)o * a.NT + tid) * (a.nh + 1);
+  const double den0 = 0.5 * a.mu;
 
   for (int h = 0; h < a.nh; ++h) {
-    for (int q = a.hoff[h] + wave; q < a.hoff[h + 1]; q += nwaves) {
-      const int packed = a.hcell[q];
-      const int oi = tile.x * a.TX + (packed & 31);
-      const int oj = tile.y * a.TY + ((packed >> 5) & 31);
-      const int ok = tile.z * a.TZ + (packed >> 10);
-      // oriented coordinate 0 is the octant's inflow layer: not swept
-      if (oi < 1 || oj < 1 || ok < 1 || oi >= a.d.Nx || oj >= a.d.Ny || ok >= a.d.Nz) continue;
-      const int i = sx > 0 ? 1 + oi : a.d.Nx - oi;
-      const int j = sy > 0 ? 1 + oj : a.d.Ny - oj;
-      const int k = sz > 0 ? 1 + ok : a.d.Nz - ok;
-      const int64_t cell = uniform64((int64_t)(i + 1) + (int64_t)(j + 1) * a.d.Nxt +
-                                     (int64_t)(k + 1) * a.d.Nxyt);
-      if (a.lab[cell] != FLUID) continue;
-      const uint32_t cc = a.code[cell];
-      const bool fx = (cc >> (3 * o)) & 1u, fy = (cc >> (3 * o + 1)) & 1u,
-                 fz = (cc >> (3 * o + 2)) & 1u;
-      const bool view = (cc >> (24 + o)) & 1u;
-      const double r = a.rho[cell], u = a.ux[cell], v = a.uy[cell], ww = a.uz[cell];
-      const double C0x = fx ? a.d1 : a.c0, C1x = fx ? a.d1 : a.c1;
-      const double C0y = fy ? a.d1 : a.c0, C1y = fy ? a.d1 : a.c1;
-      const double C0z = fz ? a.d1 : a.c0, C1z = fz ? a.d1 : a.c1;
-      const T* py1 = fo + (cell - dy) * Nc8;
-      const T* py2 = fo + (cell - 2 * dy) * Nc8;
-      const T* pz1 = fo + (cell - dz) * Nc8;
-      const T* pz2 = fo + (cell - 2 * dz) * Nc8;
-      if (view) {
-        const T* s;
-        if ((s = switched_source(a, cell - dy, o, 1))) py1 = s;
-        if (!fy && (s = switched_source(a, cell - 2 * dy, o, 1))) py2 = s;
-        if ((s = switched_source(a, cell - dz, o, 2))) pz1 = s;
-        if (!fz && (s = switched_source(a, cell - 2 * dz, o, 2))) pz2 = s;
+    const int s = off[h], e = off[h + 1];
+    for (int b0 = s + wave; b0 < e; b0 += nwaves * kBatch) {
+      int64_t cell[kBatch];
+      bool act[kBatch];
+      uint32_t cc[kBatch];
+      double r[kBatch], u[kBatch], v[kBatch], ww[kBatch];
+      double fp[kBatch], fx1[kBatch], fx2[kBatch], fy1[kBatch], fy2[kBatch], fz1[kBatch], fz2[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const int idx = b0 + k * nwaves;
+        act[k] = idx < e;
+        cell[k] = __builtin_amdgcn_readfirstlane(a.cells[act[k] ? idx : s]);
       }
-      double pr = 0.0, pu = 0.0, pv = 0.0, pw = 0.0;
-      for (int l = lane; l < Nc8; l += kWave) {
-        const double cxl = a.cx[l], cyl = a.cy[l], czl = a.cz[l];
-        double t = sx > 0 ? cxl * u : -(cxl * u);
-        t = sy > 0 ? t + cyl * v : t - cyl * v;
-        t = sz > 0 ? t + czl * ww : t - czl * ww;
-        const double feq = a.w[l] * (r + 1.0 * t);
-        T* pc = fo + cell * Nc8 + l;
-        double num = a.mu * (feq - 0.5 * (double)(*pc));
-        num = num + cxl * C1x * (double)fo[(cell - dx) * Nc8 + l];
-        if (!fx) num = num + cxl * a.c2 * (double)fo[(cell - 2 * dx) * Nc8 + l];
-        num = num + cyl * C1y * (double)py1[l];
-        if (!fy) num = num + cyl * a.c2 * (double)py2[l];
-        num = num + czl * C1z * (double)pz1[l];
-        if (!fz) num = num + czl * a.c2 * (double)pz2[l];
-        double den = 0.5 * a.mu + cxl * C0x;
-        den = den + cyl * C0y;
-        den = den + czl * C0z;
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        cc[k] = a.code[cell[k]];
+        r[k] = a.rho[cell[k]]; u[k] = a.ux[cell[k]]; v[k] = a.uy[cell[k]]; ww[k] = a.uz[cell[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const int64_t c = cell[k] * Nc8;
+        fp[k] = fo[c];
+        fx1[k] = fo[c - dx * Nc8];
+        fx2[k] = fo[c - 2 * dx * Nc8];
+        const T* py1 = fo + c - dy * Nc8;
+        const T* py2 = fo + c - 2 * dy * Nc8;
+        const T* pz1 = fo + c - dz * Nc8;
+        const T* pz2 = fo + c - 2 * dz * Nc8;
+        if ((cc[k] >> (24 + o)) & 1u) {  // reads a wall slot the reference switched
+          const T* q;
+          if ((q = switched_source(a, cell[k] - dy, o, 1))) py1 = q + ll;
+          if ((q = switched_source(a, cell[k] - 2 * dy, o, 1))) py2 = q + ll;
+          if ((q = switched_source(a, cell[k] - dz, o, 2))) pz1 = q + ll;
+          if ((q = switched_source(a, cell[k] - 2 * dz, o, 2))) pz2 = q + ll;
+        }
+        fy1[k] = *py1; fy2[k] = *py2; fz1[k] = *pz1; fz2[k] = *pz2;
+      }
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const bool fx = (cc[k] >> (3 * o)) & 1u, fy = (cc[k] >> (3 * o + 1)) & 1u,
+                   fz = (cc[k] >> (3 * o + 2)) & 1u;
+        double t = sx > 0 ? cxl * u[k] : -(cxl * u[k]);
+        t = sy > 0 ? t + cyl * v[k] : t - cyl * v[k];
+        t = sz > 0 ? t + czl * ww[k] : t - czl * ww[k];
+        const double feq = wl * (r[k] + 1.0 * t);
+        double num = a.mu * (feq - 0.5 * fp[k]);
+        num = num + cxl * (fx ? a.d1 : a.c1) * fx1[k];
+        num = num + cxl * (fx ? 0.0 : a.c2) * fx2[k];
+        num = num + cyl * (fy ? a.d1 : a.c1) * fy1[k];
+        num = num + cyl * (fy ? 0.0 : a.c2) * fy2[k];
+        num = num + czl * (fz ? a.d1 : a.c1) * fz1[k];
+        num = num + czl * (fz ? 0.0 : a.c2) * fz2[k];
+        double den = den0 + cxl * (fx ? a.d1 : a.c0);
+        den = den + cyl * (fy ? a.d1 : a.c0);
+        den = den + czl * (fz ? a.d1 : a.c0);
         const T fnew = (T)(num / den);
-        *pc = fnew;
-        pr += (double)fnew;
-        pu += cxl * (double)fnew;
-        pv += cyl * (double)fnew;
-        pw += czl * (double)fnew;
-      }
-      if (a.part) {
-        pr = wave_sum(pr);
-        pu = wave_sum(pu);
-        pv = wave_sum(pv);
-        pw = wave_sum(pw);
-        if (lane < 4) {
-          const double val = lane == 0 ? pr : lane == 1 ? sx * pu : lane == 2 ? sy * pv : sz * pw;
-          a.part[((size_t)cell * 8 + o) * 4 + lane] = val;
+        if (act[k] && lact) fo[cell[k] * Nc8] = fnew;
+        if (a.part) {
+          const double fv = lact ? (double)fnew : 0.0;
+          const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv, lane);
+          if (act[k] && (lane & 15) == 0) {
+            const int q = lane >> 4;
+            const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
+            a.part[(((size_t)cell[k] * 8 + o) * nch + ch) * 4 + q] = sgn * z;
+          }
         }
       }
     }
@@ -395,13 +426,14 @@ __global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
   wall_dispatch<T, false>(a, idx, threadIdx.x & (kWave - 1));
 }
 
-// walls that read another wall: one wave, in the reference's wall order
+// walls that read another wall, one dependency level per launch: reads of
+// earlier walls see their new values (lower levels ran before), reads of later
+// walls come from the snapshot taken before the wall pass
 template <typename T>
-__global__ __launch_bounds__(64) void k_walls_serial(WallArgs<T> a) {
-  for (int idx = 0; idx < a.n; ++idx) {
-    wall_dispatch<T, true>(a, idx, threadIdx.x);
-    __syncthreads();
-  }
+__global__ __launch_bounds__(256) void k_walls_coupled(WallArgs<T> a) {
+  const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (idx >= a.n) return;
+  wall_dispatch<T, true>(a, idx, threadIdx.x & (kWave - 1));
 }
 
 template <typename T>
@@ -484,19 +516,18 @@ __global__ void k_sym_z(T* f, Dims d) {
 
 // Interior fluid cells: their f is final after the sweep, so the moments are
 // the sum of the eight octant partials.
-__global__ void k_moments_partials(const double* part, const int32_t* cells, int64_t n,
+__global__ void k_moments_partials(const double* part, int nchunk, const int32_t* cells, int64_t n,
                                    double* rho, double* ux, double* uy, double* uz) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = cells[t];
-    const double* p = part + (size_t)c * 32;
+    const double* p = part + (size_t)c * 32 * nchunk;
     double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      r += p[o * 4 + 0];
-      x += p[o * 4 + 1];
-      y += p[o * 4 + 2];
-      z += p[o * 4 + 3];
+    for (int q = 0; q < 8 * nchunk; ++q) {
+      r += p[q * 4 + 0];
+      x += p[q * 4 + 1];
+      y += p[q * 4 + 2];
+      z += p[q * 4 + 3];
     }
     rho[c] = r; ux[c] = x; uy[c] = y; uz[c] = z;
   }
@@ -597,14 +628,13 @@ struct Ctx {
   int32_t *uncoupled = nullptr, *coupled = nullptr, *coupled_snap = nullptr, *snap_cell = nullptr;
   int32_t *commit_walls = nullptr;
   int32_t *fluid_int = nullptr, *fluid_bnd = nullptr, *fluid_all = nullptr;
-  // tiles
-  int TX = 8, TY = 8, TZ = 8;
+  // sweep schedule: tiles grouped by diagonal I+J+K (oriented)
   int ndiag = 0;
+  int nchunk = 1;              // 64-velocity chunks
   std::vector<int> diag_off;   // ndiag + 1
   int4* tiles = nullptr;
-  uint16_t* hcell = nullptr;
-  int32_t* hoff = nullptr;
-  int nh = 0;
+  int32_t* sweep_cells = nullptr;
+  int32_t* sweep_off = nullptr;
   // timing: four events per step of the last piks_iterate call
   std::vector<hipEvent_t> ev;
   double last_ms[4] = {0, 0, 0, 0};
@@ -671,37 +701,24 @@ int setup_velocity(Ctx& c) {
   return 0;
 }
 
-// in-tile cells grouped by hyperplane a+b+c, and tiles grouped by diagonal
+// tiles of the oriented grid grouped by diagonal I+J+K (one launch each)
 int setup_tiles(Ctx& c) {
-  const int TX = c.TX, TY = c.TY, TZ = c.TZ;
-  std::vector<uint16_t> hc;
-  std::vector<int32_t> ho;
-  c.nh = TX + TY + TZ - 2;
-  for (int h = 0; h < c.nh; ++h) {
-    ho.push_back((int32_t)hc.size());
-    for (int cc = 0; cc < TZ; ++cc)
-      for (int b = 0; b < TY; ++b) {
-        const int a = h - b - cc;
-        if (a >= 0 && a < TX) hc.push_back((uint16_t)(a | (b << 5) | (cc << 10)));
-      }
-  }
-  ho.push_back((int32_t)hc.size());
-  const int NTX = (c.g.Nx + TX - 1) / TX, NTY = (c.g.Ny + TY - 1) / TY, NTZ = (c.g.Nz + TZ - 1) / TZ;
-  c.ndiag = NTX + NTY + NTZ - 2;
+  const Geometry& g = c.g;
+  c.ndiag = g.NTX + g.NTY + g.NTZ - 2;
   std::vector<int4> tl;
   c.diag_off.clear();
   for (int dg = 0; dg < c.ndiag; ++dg) {
     c.diag_off.push_back((int)tl.size());
-    for (int K = 0; K < NTZ; ++K)
-      for (int J = 0; J < NTY; ++J) {
+    for (int K = 0; K < g.NTZ; ++K)
+      for (int J = 0; J < g.NTY; ++J) {
         const int I = dg - J - K;
-        if (I >= 0 && I < NTX) tl.push_back(make_int4(I, J, K, 0));
+        if (I >= 0 && I < g.NTX) tl.push_back(make_int4(I, J, K, 0));
       }
   }
   c.diag_off.push_back((int)tl.size());
-  if (int rc = upload(c, &c.hcell, hc)) return rc;
-  if (int rc = upload(c, &c.hoff, ho)) return rc;
   if (int rc = upload(c, &c.tiles, tl)) return rc;
+  if (int rc = upload(c, &c.sweep_cells, g.sweep_cells)) return rc;
+  if (int rc = upload(c, &c.sweep_off, g.sweep_off)) return rc;
   return 0;
 }
 
@@ -724,13 +741,15 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   if (int rc = build_geometry(gi, raw, nbytes, c->g, err)) return fail(rc, err);
   const Geometry& g = c->g;
   c->d = Dims{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, g.Ntotal, (int)c->cx.size()};
+  c->nchunk = (c->d.Nc8 + kWave - 1) / kWave;
   c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
   const size_t nf = (size_t)g.Ntotal * 8 * c->d.Nc8;
   size_t free_b = 0, total_b = 0;
   HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-  const size_t need = nf * sizeof(double) + (size_t)g.Ntotal * (32 * 8 + 4 * 8 + 16);
+  const size_t need = nf * sizeof(double) + (size_t)g.Ntotal * (32 * 8 * c->nchunk + 4 * 8 + 16) +
+                      g.sweep_cells.size() * 4;
   if (need > free_b)
     return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
                                 std::to_string(free_b >> 20) + " MiB free");
@@ -741,7 +760,7 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   if ((rc = dalloc(*c, &c->ux, (size_t)g.Ntotal))) return rc;
   if ((rc = dalloc(*c, &c->uy, (size_t)g.Ntotal))) return rc;
   if ((rc = dalloc(*c, &c->uz, (size_t)g.Ntotal))) return rc;
-  if ((rc = dalloc(*c, &c->part, (size_t)g.Ntotal * 32))) return rc;
+  if ((rc = dalloc(*c, &c->part, (size_t)g.Ntotal * 32 * c->nchunk))) return rc;
   const size_t nb = (size_t)g.Ny * g.Nz * 8 * c->d.Nc8;
   if ((rc = dalloc(*c, &c->buf_e, nb))) return rc;
   if ((rc = dalloc(*c, &c->buf_w, nb))) return rc;
@@ -787,14 +806,16 @@ int iterate(Ctx& c, int nsteps) {
   const Dims d = c.d;
   SweepArgs<double> sa{};
   sa.f = c.f; sa.lab = c.lab; sa.code = c.code;
+  sa.cells = c.sweep_cells; sa.off = c.sweep_off;
+  sa.NTY = g.NTY; sa.NTZ = g.NTZ; sa.nh = g.nh; sa.nchunk = c.nchunk;
+  sa.NT = (int64_t)g.NTX * g.NTY * g.NTZ;
   sa.rho = c.rho; sa.ux = c.ux; sa.uy = c.uy; sa.uz = c.uz;
   sa.part = c.exact ? nullptr : c.part;
   sa.wid = c.wid; sa.views = c.views; sa.wall_corner = c.wall_corner; sa.fw = c.fw;
   sa.cx = c.dcx; sa.cy = c.dcy; sa.cz = c.dcz; sa.w = c.dw;
   sa.mu = c.mu;
   sa.c0 = 1.5 / g.ds; sa.c1 = 2.0 / g.ds; sa.c2 = -0.5 / g.ds; sa.d1 = 1.0 / g.ds;
-  sa.d = d; sa.TX = c.TX; sa.TY = c.TY; sa.TZ = c.TZ;
-  sa.hcell = c.hcell; sa.hoff = c.hoff; sa.nh = c.nh;
+  sa.d = d;
   const int wpb = 4;  // waves per block for one-wave-per-item kernels
   const int timed = std::min(nsteps, kMaxTimedSteps);
   while ((int)c.ev.size() < 4 * timed) {
@@ -808,7 +829,7 @@ int iterate(Ctx& c, int nsteps) {
     for (int dg = 0; dg < c.ndiag; ++dg) {
       const int nt = c.diag_off[dg + 1] - c.diag_off[dg];
       sa.tiles = c.tiles + c.diag_off[dg];
-      hipLaunchKernelGGL(k_sweep<double>, dim3(nt * 8), dim3(256), 0, c.stream, sa);
+      hipLaunchKernelGGL(k_sweep<double>, dim3(nt * 8 * c.nchunk), dim3(256), 0, c.stream, sa);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
@@ -826,9 +847,12 @@ int iterate(Ctx& c, int nsteps) {
       hipLaunchKernelGGL(k_walls<double>, dim3(grid_for((int64_t)g.uncoupled.size(), wpb)),
                          dim3(wpb * kWave), 0, c.stream, wa);
     }
-    if (!g.coupled.empty()) {
-      WallArgs<double> wa = wall_args<double>(c, c.coupled, (int)g.coupled.size());
-      hipLaunchKernelGGL(k_walls_serial<double>, dim3(1), dim3(kWave), 0, c.stream, wa);
+    for (size_t lv = 0; lv + 1 < g.coupled_level_off.size(); ++lv) {
+      const int b = g.coupled_level_off[lv], n = g.coupled_level_off[lv + 1] - b;
+      WallArgs<double> wa = wall_args<double>(c, c.coupled + b, n);
+      wa.snap_idx = c.coupled_snap + (size_t)b * 6;
+      hipLaunchKernelGGL(k_walls_coupled<double>, dim3(grid_for(n, wpb)), dim3(wpb * kWave), 0,
+                         c.stream, wa);
     }
     hipLaunchKernelGGL(k_inout<double>, dim3(grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256)), dim3(256),
                        0, c.stream, c.f, (const double*)c.dw, c.prm.press_drop, d);
@@ -846,7 +870,7 @@ int iterate(Ctx& c, int nsteps) {
       const int64_t ni = (int64_t)g.fluid_interior.size(), nbd = (int64_t)g.fluid_boundary.size();
       if (ni > 0)
         hipLaunchKernelGGL(k_moments_partials, dim3(grid_for(ni, 256)), dim3(256), 0, c.stream,
-                           (const double*)c.part, c.fluid_int, ni, c.rho, c.ux, c.uy, c.uz);
+                           (const double*)c.part, c.nchunk, c.fluid_int, ni, c.rho, c.ux, c.uy, c.uz);
       if (nbd > 0)
         hipLaunchKernelGGL(k_moments_direct<double>, dim3((unsigned)((nbd + wpb - 1) / wpb)),
                            dim3(wpb * kWave), 0, c.stream, (const double*)c.f, c.fluid_bnd, nbd,
@@ -1007,6 +1031,30 @@ int piks_get_info(void* ctx, piks_info* info) {
   info->device_bytes = c.bytes;
   info->coupled_walls = (int32_t)g.coupled.size();
   info->precision = c.prm.precision;
+  return 0;
+}
+
+int piks_preprocess(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
+                    int32_t* labels, piks_info* info) {
+  g_err.clear();
+  if (!p || !raw_image) return fail(PIKS_E_ARG, "null argument");
+  Geometry g;
+  std::string err;
+  GeomInput gi{p->nx, p->ny, p->nz, p->fluid_layer, p->wall_ext_order};
+  if (int rc = build_geometry(gi, raw_image, image_bytes, g, err)) return fail(rc, err);
+  if (labels)
+    for (size_t q = 0; q < g.lab.size(); ++q) labels[q] = g.lab[q];
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    info->nx = g.Nx; info->ny = g.Ny; info->nz = g.Nz;
+    info->nxt = g.Nxt; info->nyt = g.Nyt; info->nzt = g.Nzt;
+    info->ntotal = g.Ntotal;
+    info->nwall = (int32_t)g.wall_cell.size();
+    info->ncorner = g.ncorner;
+    info->nfluid = g.nfluid;
+    info->ds = g.ds;
+    info->coupled_walls = (int32_t)g.coupled.size();
+  }
   return 0;
 }
 
