@@ -86,7 +86,7 @@ def gaussian_field(n: int, porosity: float, corr_len: float, seed: int,
     kx = np.fft.rfftfreq(nx)[None, None, :]
     # covariance exp(-r^2/l^2)  <=>  filter exp(-(pi l k)^2 / 2) on the noise
     filt = np.exp(-0.5 * (np.pi * corr_len) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2))
-    field = np.fft.irfftn(np.fft.rfftn(noise) * filt, s=(nz, ny, nx))
+    field = np.fft.irfftn(np.fft.rfftn(noise) * filt, s=(nz, ny, nx), axes=(0, 1, 2))
     thr = np.quantile(field, porosity)
     pore = field < thr
     if percolate_x:
