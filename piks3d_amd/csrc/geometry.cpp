@@ -343,12 +343,15 @@ void build_sweep_schedule(Geometry& g, int tx, int ty, int tz) {
   g.sweep_off.assign((size_t)(8 * NT * nh1), 0);
   std::vector<int64_t> key;     // bucket of each swept cell, per octant
   std::vector<int32_t> cells;
+  std::vector<uint16_t> local;
   g.sweep_cells.clear();
+  g.sweep_local.clear();
   for (int o = 0; o < 8; ++o) {
     const int sx = osx(o), sy = osy(o), sz = osz(o);
     std::vector<int32_t> count((size_t)(NT * g.nh), 0);
     key.clear();
     cells.clear();
+    local.clear();
     for (int k = 1; k <= g.Nz; ++k)
       for (int j = 1; j <= g.Ny; ++j)
         for (int i = 1; i <= g.Nx; ++i) {
@@ -363,6 +366,7 @@ void build_sweep_schedule(Geometry& g, int tx, int ty, int tz) {
           const int64_t b = t * g.nh + h;
           key.push_back(b);
           cells.push_back((int32_t)c);
+          local.push_back((uint16_t)(oi % tx | (oj % ty) << 5 | (ok % tz) << 10));
           ++count[(size_t)b];
         }
     // counting sort by (tile, hyperplane); within a bucket the raster order is kept
@@ -370,8 +374,13 @@ void build_sweep_schedule(Geometry& g, int tx, int ty, int tz) {
     for (size_t b = 0; b < count.size(); ++b) start[b + 1] = start[b] + count[b];
     const int64_t base = (int64_t)g.sweep_cells.size();
     g.sweep_cells.resize((size_t)(base + (int64_t)cells.size()));
+    g.sweep_local.resize(g.sweep_cells.size());
     std::vector<int64_t> pos(start.begin(), start.end() - 1);
-    for (size_t q = 0; q < cells.size(); ++q) g.sweep_cells[(size_t)(base + pos[(size_t)key[q]]++)] = cells[q];
+    for (size_t q = 0; q < cells.size(); ++q) {
+      const size_t at = (size_t)(base + pos[(size_t)key[q]]++);
+      g.sweep_cells[at] = cells[q];
+      g.sweep_local[at] = local[q];
+    }
     for (int64_t t = 0; t < NT; ++t) {
       int32_t* off = &g.sweep_off[(size_t)(((int64_t)o * NT + t) * nh1)];
       for (int h = 0; h < g.nh; ++h) off[h] = (int32_t)(base + start[(size_t)(t * g.nh + h)]);

@@ -61,6 +61,7 @@ struct Geometry {
   // sweep_off[((o*NT + t)*(nh+1)) + h] is where hyperplane h of tile t starts.
   int TX = 8, TY = 8, TZ = 8, NTX = 0, NTY = 0, NTZ = 0, nh = 0;
   std::vector<int32_t> sweep_cells;
+  std::vector<uint16_t> sweep_local;   // in-tile oriented (a | b << 5 | c << 10) of each entry
   std::vector<int32_t> sweep_off;
 
   int64_t cell(int i, int j, int k) const {

@@ -95,12 +95,19 @@ struct SweepArgs {
   Dims d;
   const int4* tiles;       // oriented tile coordinates of this diagonal
   const int32_t* cells;    // swept cells per (octant, tile), by in-tile hyperplane
+  const uint16_t* local;   // their in-tile oriented coordinates a | b << 5 | c << 10
   const int32_t* off;      // hyperplane offsets into `cells`
   int NTY, NTZ, nh, nchunk;
   int64_t NT;
 };
 
-constexpr int kBatch = 4;  // cells whose loads a wave has in flight together
+// Sweep tiles are kTile^3 cells of one octant's oriented grid.
+constexpr int kTile = 8;
+constexpr int kTileH = 3 * kTile - 2;            // in-tile hyperplanes a+b+c
+constexpr int kTileHMax = 48;                    // most cells on one hyperplane of an 8^3 tile
+constexpr int kTileCells = kTile * kTile * kTile;
+constexpr int kRound = 2;                        // cells a wave updates together
+constexpr int kSweepThreads = 256;
 
 // Value a cell reading neighbour n along `axis` (1 = y, 2 = z) sees in octant
 // o, when the reference would have switched that wall slot by then; null if
@@ -133,14 +140,36 @@ __device__ __forceinline__ double wave_sum4(double a, double b, double c, double
   return z;
 }
 
+// Everything one update needs that does not depend on this tile's progress:
+// the cell, its flags, its own previous f, and the upstream neighbours that
+// are not updated in this tile (other tiles, walls, inflow layers).
+template <typename T>
+struct Pending {
+  int e;             // entry in the tile's list, -1 if this slot is idle
+  int32_t cell;
+  uint32_t code;
+  double r, u, v, w; // moments of the previous iteration at the cell
+  T fp;
+  T nb[6];           // x1 x2 y1 y2 z1 z2 when read from memory
+  int16_t ring[6];   // LDS ring slot of the neighbour, -1 when nb[] holds it
+};
+
 // One octant's sweep over one tile for one chunk of 64 velocities.  Cells of
-// in-tile hyperplane h depend only on hyperplanes h-1, h-2 and on upstream
-// tiles (earlier launches), so a barrier per hyperplane orders the tile.
+// in-tile hyperplane h depend only on hyperplanes h-1, h-2 of this tile and on
+// upstream tiles (earlier launches); a barrier per hyperplane orders the tile
+// and the new values of the last three hyperplanes stay in LDS.
 // Each update is src/solver.F90:206-214 (and analogues) with the same
 // operations in the same order (contraction off): signs enter as exact
 // negations, and a first-order axis contributes (c*0)*f like the reference.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<T> a) {
+  __shared__ T ring[3 * kTileHMax * kWave];
+  __shared__ int32_t s_cell[kTileCells];
+  __shared__ uint32_t s_code[kTileCells];
+  __shared__ uint16_t s_local[kTileCells];
+  __shared__ int16_t s_pos[kTileCells];
+  __shared__ int32_t s_hoff[kTileH + 1];
+
   const int nch = a.nchunk;
   const int ch = blockIdx.x % nch;
   const int o = (blockIdx.x / nch) & 7;
@@ -154,80 +183,137 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs<T> a) {
   const int ll = lact ? l : 0;
   const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
   const int sx = osx(o), sy = osy(o), sz = osz(o);
-  const int64_t dx = sx, dy = (int64_t)sy * a.d.Nxt, dz = (int64_t)sz * a.d.Nxyt;
+  const int64_t step[3] = {(int64_t)sx, (int64_t)sy * a.d.Nxt, (int64_t)sz * a.d.Nxyt};
   T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8 + ll;
   const int64_t tid = ((int64_t)tile.x * a.NTY + tile.y) * a.NTZ + tile.z;
-  const int32_t* off = a.off + ((int64_t)o * a.NT + tid) * (a.nh + 1);
+  const int32_t* off = a.off + ((int64_t)o * a.NT + tid) * (kTileH + 1);
+  const int base = off[0];
+
+  // stage the tile's update list
+  for (int i = threadIdx.x; i <= kTileH; i += blockDim.x) s_hoff[i] = off[i] - base;
+  for (int i = threadIdx.x; i < kTileCells; i += blockDim.x) s_pos[i] = -1;
+  __syncthreads();
+  const int n = s_hoff[kTileH];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int32_t c = a.cells[base + i];
+    const uint16_t loc = a.local[base + i];
+    s_cell[i] = c;
+    s_code[i] = a.code[c];
+    s_local[i] = loc;
+    const int la = loc & 31, lb = (loc >> 5) & 31, lc = loc >> 10;
+    s_pos[(lc * kTile + lb) * kTile + la] = (int16_t)(i - s_hoff[la + lb + lc]);
+  }
+  __syncthreads();
+
+  const int per_round = nwaves * kRound;
+  auto rounds = [&](int h) { return (s_hoff[h + 1] - s_hoff[h] + per_round - 1) / per_round; };
+
+  // loads for round r of hyperplane h into p
+  auto prefetch = [&](Pending<T>* p, int h, int r) {
+    const int hs = s_hoff[h], he = s_hoff[h + 1];
+#pragma unroll
+    for (int k = 0; k < kRound; ++k) {
+      const int e = hs + (r * kRound + k) * nwaves + wave;
+      const int es = e < he ? e : hs;
+      p[k].e = e < he ? e : -1;
+      const int32_t cell = s_cell[es];
+      const uint32_t cc = s_code[es];
+      const uint16_t loc = s_local[es];
+      p[k].cell = cell;
+      p[k].code = cc;
+      p[k].r = a.rho[cell]; p[k].u = a.ux[cell]; p[k].v = a.uy[cell]; p[k].w = a.uz[cell];
+      p[k].fp = fo[(int64_t)cell * Nc8];
+      const int lc3[3] = {loc & 31, (loc >> 5) & 31, loc >> 10};
+      const bool view = (cc >> (24 + o)) & 1u;
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        const int ax = d >> 1, dist = (d & 1) + 1;
+        int q[3] = {lc3[0], lc3[1], lc3[2]};
+        q[ax] -= dist;
+        int slot = -1;
+        if (q[ax] >= 0) {
+          const int ps = s_pos[(q[2] * kTile + q[1]) * kTile + q[0]];
+          if (ps >= 0) slot = ((h - dist) % 3) * kTileHMax + ps;
+        }
+        p[k].ring[d] = (int16_t)slot;
+        if (slot < 0) {
+          const int64_t nbr = (int64_t)cell - dist * step[ax];
+          const T* src = fo + nbr * Nc8;
+          if (view && ax > 0) {
+            const T* q2 = switched_source(a, nbr, o, ax);
+            if (q2) src = q2 + ll;
+          }
+          p[k].nb[d] = *src;
+        } else {
+          p[k].nb[d] = (T)0;
+        }
+      }
+    }
+  };
+
+  // first round that has cells, from hyperplane h on
+  auto first_from = [&](int h, int* hh) {
+    for (; h < kTileH; ++h)
+      if (s_hoff[h + 1] > s_hoff[h]) { *hh = h; return true; }
+    return false;
+  };
+
+  Pending<T> cur[kRound], nxt[kRound];
+  int h0 = 0;
+  if (first_from(0, &h0)) prefetch(cur, h0, 0);
   const double den0 = 0.5 * a.mu;
 
-  for (int h = 0; h < a.nh; ++h) {
-    const int s = off[h], e = off[h + 1];
-    for (int b0 = s + wave; b0 < e; b0 += nwaves * kBatch) {
-      int64_t cell[kBatch];
-      bool act[kBatch];
-      uint32_t cc[kBatch];
-      double r[kBatch], u[kBatch], v[kBatch], ww[kBatch];
-      double fp[kBatch], fx1[kBatch], fx2[kBatch], fy1[kBatch], fy2[kBatch], fz1[kBatch], fz2[kBatch];
+  for (int h = 0; h < kTileH; ++h) {
+    const int nr = rounds(h);
+    T* rw = ring + (h % 3) * kTileHMax * kWave + lane;
+    for (int r = 0; r < nr; ++r) {
+      // issue the next round's loads before working on this one
+      int hn = h, rn = r + 1;
+      bool more = rn < nr;
+      if (!more) { rn = 0; more = first_from(h + 1, &hn); }
+      if (more) prefetch(nxt, hn, rn);
 #pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        const int idx = b0 + k * nwaves;
-        act[k] = idx < e;
-        cell[k] = __builtin_amdgcn_readfirstlane(a.cells[act[k] ? idx : s]);
-      }
+      for (int k = 0; k < kRound; ++k) {
+        const Pending<T>& p = cur[k];
+        double nbv[6];
 #pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        cc[k] = a.code[cell[k]];
-        r[k] = a.rho[cell[k]]; u[k] = a.ux[cell[k]]; v[k] = a.uy[cell[k]]; ww[k] = a.uz[cell[k]];
-      }
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        const int64_t c = cell[k] * Nc8;
-        fp[k] = fo[c];
-        fx1[k] = fo[c - dx * Nc8];
-        fx2[k] = fo[c - 2 * dx * Nc8];
-        const T* py1 = fo + c - dy * Nc8;
-        const T* py2 = fo + c - 2 * dy * Nc8;
-        const T* pz1 = fo + c - dz * Nc8;
-        const T* pz2 = fo + c - 2 * dz * Nc8;
-        if ((cc[k] >> (24 + o)) & 1u) {  // reads a wall slot the reference switched
-          const T* q;
-          if ((q = switched_source(a, cell[k] - dy, o, 1))) py1 = q + ll;
-          if ((q = switched_source(a, cell[k] - 2 * dy, o, 1))) py2 = q + ll;
-          if ((q = switched_source(a, cell[k] - dz, o, 2))) pz1 = q + ll;
-          if ((q = switched_source(a, cell[k] - 2 * dz, o, 2))) pz2 = q + ll;
-        }
-        fy1[k] = *py1; fy2[k] = *py2; fz1[k] = *pz1; fz2[k] = *pz2;
-      }
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        const bool fx = (cc[k] >> (3 * o)) & 1u, fy = (cc[k] >> (3 * o + 1)) & 1u,
-                   fz = (cc[k] >> (3 * o + 2)) & 1u;
-        double t = sx > 0 ? cxl * u[k] : -(cxl * u[k]);
-        t = sy > 0 ? t + cyl * v[k] : t - cyl * v[k];
-        t = sz > 0 ? t + czl * ww[k] : t - czl * ww[k];
-        const double feq = wl * (r[k] + 1.0 * t);
-        double num = a.mu * (feq - 0.5 * fp[k]);
-        num = num + cxl * (fx ? a.d1 : a.c1) * fx1[k];
-        num = num + cxl * (fx ? 0.0 : a.c2) * fx2[k];
-        num = num + cyl * (fy ? a.d1 : a.c1) * fy1[k];
-        num = num + cyl * (fy ? 0.0 : a.c2) * fy2[k];
-        num = num + czl * (fz ? a.d1 : a.c1) * fz1[k];
-        num = num + czl * (fz ? 0.0 : a.c2) * fz2[k];
+        for (int d = 0; d < 6; ++d)
+          nbv[d] = p.ring[d] >= 0 ? (double)ring[p.ring[d] * kWave + lane] : (double)p.nb[d];
+        const uint32_t cc = p.code;
+        const bool fx = (cc >> (3 * o)) & 1u, fy = (cc >> (3 * o + 1)) & 1u,
+                   fz = (cc >> (3 * o + 2)) & 1u;
+        const double r0 = p.r, u = p.u, v = p.v, ww = p.w;
+        double t = sx > 0 ? cxl * u : -(cxl * u);
+        t = sy > 0 ? t + cyl * v : t - cyl * v;
+        t = sz > 0 ? t + czl * ww : t - czl * ww;
+        const double feq = wl * (r0 + 1.0 * t);
+        double num = a.mu * (feq - 0.5 * (double)p.fp);
+        num = num + cxl * (fx ? a.d1 : a.c1) * nbv[0];
+        num = num + cxl * (fx ? 0.0 : a.c2) * nbv[1];
+        num = num + cyl * (fy ? a.d1 : a.c1) * nbv[2];
+        num = num + cyl * (fy ? 0.0 : a.c2) * nbv[3];
+        num = num + czl * (fz ? a.d1 : a.c1) * nbv[4];
+        num = num + czl * (fz ? 0.0 : a.c2) * nbv[5];
         double den = den0 + cxl * (fx ? a.d1 : a.c0);
         den = den + cyl * (fy ? a.d1 : a.c0);
         den = den + czl * (fz ? a.d1 : a.c0);
         const T fnew = (T)(num / den);
-        if (act[k] && lact) fo[cell[k] * Nc8] = fnew;
+        if (p.e >= 0) {
+          if (lact) fo[(int64_t)p.cell * Nc8] = fnew;
+          rw[(p.e - s_hoff[h]) * kWave] = fnew;
+        }
         if (a.part) {
           const double fv = lact ? (double)fnew : 0.0;
           const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv, lane);
-          if (act[k] && (lane & 15) == 0) {
+          if (p.e >= 0 && (lane & 15) == 0) {
             const int q = lane >> 4;
             const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
-            a.part[(((size_t)cell[k] * 8 + o) * nch + ch) * 4 + q] = sgn * z;
+            a.part[(((size_t)p.cell * 8 + o) * nch + ch) * 4 + q] = sgn * z;
           }
         }
       }
+#pragma unroll
+      for (int k = 0; k < kRound; ++k) cur[k] = nxt[k];
     }
     __syncthreads();
   }
@@ -634,6 +720,7 @@ struct Ctx {
   std::vector<int> diag_off;   // ndiag + 1
   int4* tiles = nullptr;
   int32_t* sweep_cells = nullptr;
+  uint16_t* sweep_local = nullptr;
   int32_t* sweep_off = nullptr;
   // timing: four events per step of the last piks_iterate call
   std::vector<hipEvent_t> ev;
@@ -718,6 +805,7 @@ int setup_tiles(Ctx& c) {
   c.diag_off.push_back((int)tl.size());
   if (int rc = upload(c, &c.tiles, tl)) return rc;
   if (int rc = upload(c, &c.sweep_cells, g.sweep_cells)) return rc;
+  if (int rc = upload(c, &c.sweep_local, g.sweep_local)) return rc;
   if (int rc = upload(c, &c.sweep_off, g.sweep_off)) return rc;
   return 0;
 }
@@ -781,6 +869,8 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   std::vector<int32_t> all(g.fluid_interior);
   all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
   if ((rc = upload(*c, &c->fluid_all, all))) return rc;
+  if (g.TX != kTile || g.TY != kTile || g.TZ != kTile || g.nh != kTileH)
+    return fail(PIKS_E_STATE, "sweep schedule does not match the kernel's tile");
   if ((rc = setup_tiles(*c))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c.release();
@@ -806,7 +896,7 @@ int iterate(Ctx& c, int nsteps) {
   const Dims d = c.d;
   SweepArgs<double> sa{};
   sa.f = c.f; sa.lab = c.lab; sa.code = c.code;
-  sa.cells = c.sweep_cells; sa.off = c.sweep_off;
+  sa.cells = c.sweep_cells; sa.local = c.sweep_local; sa.off = c.sweep_off;
   sa.NTY = g.NTY; sa.NTZ = g.NTZ; sa.nh = g.nh; sa.nchunk = c.nchunk;
   sa.NT = (int64_t)g.NTX * g.NTY * g.NTZ;
   sa.rho = c.rho; sa.ux = c.ux; sa.uy = c.uy; sa.uz = c.uz;
@@ -829,7 +919,7 @@ int iterate(Ctx& c, int nsteps) {
     for (int dg = 0; dg < c.ndiag; ++dg) {
       const int nt = c.diag_off[dg + 1] - c.diag_off[dg];
       sa.tiles = c.tiles + c.diag_off[dg];
-      hipLaunchKernelGGL(k_sweep<double>, dim3(nt * 8 * c.nchunk), dim3(256), 0, c.stream, sa);
+      hipLaunchKernelGGL(k_sweep<double>, dim3(nt * 8 * c.nchunk), dim3(kSweepThreads), 0, c.stream, sa);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
