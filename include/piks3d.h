@@ -96,6 +96,16 @@ int piks_create(const piks_params* p, const uint8_t* raw_image, int64_t image_by
  * called from src/main.F90:100). */
 int piks_iterate(void* ctx, int32_t nsteps);
 
+/* Precision the next iterations run in, for a context created with
+ * PIKS_PREC_FP32_REFINED (a PIKS_PREC_FP64 context accepts only FP64).  The
+ * carried state (f, fw and the x exchange planes) is converted on the device;
+ * moments, wall densities and the mass-flow check stay FP64 throughout.  The
+ * refined solve runs FP32 iterations until the check's relative change
+ * stalls, then switches to FP64 for the last iterations, so its converged
+ * permeability is the FP64 solve's.  No counterpart in the reference, which
+ * is FP64 only. */
+int piks_set_precision(void* ctx, int32_t precision);
+
 /* Convergence check; replaces chkConverge (src/output.F90:60-164):
  * out[0] = mass flow, out[1] = apparent permeability, out[2] = error, and the
  * stored mass becomes out[0], exactly like the reference. */

@@ -62,6 +62,7 @@ _API = {
     "piks_create": (ctypes.c_int, [ctypes.POINTER(PiksParams), ctypes.c_void_p, ctypes.c_int64,
                                    ctypes.POINTER(ctypes.c_void_p)]),
     "piks_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "piks_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "piks_chk_converge": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_mass_flow": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_get_macro": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
@@ -175,6 +176,11 @@ class Solver:
     def iterate(self, n=1):
         _check(lib().piks_iterate(self._h, int(n)))
 
+    def set_precision(self, precision):
+        code = {"fp64": PIKS_PREC_FP64, "fp32": PIKS_PREC_FP32_REFINED}.get(precision, precision)
+        _check(lib().piks_set_precision(self._h, int(code)))
+        self.info = self.get_info()
+
     def chk_converge(self):
         out = (ctypes.c_double * 3)()
         _check(lib().piks_chk_converge(self._h, out))
@@ -214,27 +220,54 @@ class Solver:
         _check(lib().piks_last_timing(self._h, ms))
         return list(ms)
 
-    def solve(self, max_step=None, chk_step=None, eps=None, callback=None):
+    def solve(self, max_step=None, chk_step=None, eps=None, callback=None, refine_switch=None):
         """The reference's main loop (src/main.F90:95-121): returns a dict with
-        steps, permeability, mass, error and the check history."""
+        steps, permeability, mass, error and the check history.
+
+        A context created with precision 'fp32' runs FP32 iterations while
+        the check's error is above refine_switch (and still falling), then
+        switches to FP64 for the remaining iterations; `steps32` in the result
+        counts the FP32 ones."""
         p = self.params
         max_step = p.max_step if max_step is None else max_step
         chk = p.chk_converge_step if chk_step is None else chk_step
         eps = p.eps if eps is None else eps
+        tau = p.refine_switch if refine_switch is None else refine_switch
+        refined = self.info.precision == PIKS_PREC_FP32_REFINED
+        steps32 = 0
+        best, stalled = float("inf"), 0
         error = 1.0
         perm = 0.0
         step = 0
-        for step in range(1, max_step + 1):
-            self.iterate(1)
+        while step < max_step:
+            # iterate to the next check (or the end) in one call, like the
+            # reference's loop does one iteration at a time
+            n = min(chk - step % chk, max_step - step)
+            self.iterate(n)
+            step += n
+            if refined:
+                steps32 += n
             if step % chk == 0:
                 m, perm, error = self.chk_converge()
                 self.history.append((step, m, perm, error))
                 if callback:
                     callback(step, m, perm, error)
+            if refined:
+                # FP32 has converged as far as it can once the error is below
+                # the switch level or has not improved for three checks
+                if error < best:
+                    best, stalled = error, 0
+                else:
+                    stalled += 1
+                if error <= max(tau, eps) or stalled >= 3:
+                    self.set_precision("fp64")
+                    refined = False
+                    error = 1.0
+                continue
             if error <= eps:
                 if step < chk:
                     m, perm, error = self.chk_converge()
                     self.history.append((step, m, perm, error))
                 break
-        return dict(steps=step, permeability=perm, mass=self.mass, error=error,
+        return dict(steps=step, steps32=steps32, permeability=perm, mass=self.mass, error=error,
                     history=list(self.history))

@@ -684,6 +684,26 @@ struct DevBuf {
   size_t n = 0;
 };
 
+// Device state that carries from one iteration to the next, in one precision.
+template <typename T>
+struct State {
+  T* f = nullptr;        // [octant][cell][velocity]
+  T* fw = nullptr;       // [corner][octant][velocity]
+  T* buf_e = nullptr;    // x = Nx plane after the previous sweep (ghost x = 0 next time)
+  T* buf_w = nullptr;    // x = 1 plane after the previous sweep (ghost x = Nx+1 next time)
+  T* snap = nullptr;     // walls read by earlier coupled walls, before the wall pass
+};
+
+template <typename T>
+__global__ void k_convert(T* dst, const double* src, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = (T)src[t];
+}
+__global__ void k_widen(double* dst, const float* src, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = (double)src[t];
+}
+
 struct Ctx {
   piks_params prm{};
   Geometry g;
@@ -695,13 +715,13 @@ struct Ctx {
   hipStream_t stream = nullptr;
   int device = 0;
   int64_t bytes = 0;
-  // device state
-  double* f = nullptr;
-  double* fw = nullptr;
+  // device state: FP64 always; FP32 too for PIKS_PREC_FP32_REFINED contexts
+  State<double> s64;
+  State<float> s32;
+  int phase = PIKS_PREC_FP64;   // precision the next iterations run in
+  size_t n_f = 0, n_fw = 0, n_buf = 0, n_snap = 0;
   double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
   double* part = nullptr;
-  double *buf_e = nullptr, *buf_w = nullptr;
-  double* snap = nullptr;
   double* plane = nullptr;
   double *dcx = nullptr, *dcy = nullptr, *dcz = nullptr, *dw = nullptr;
   uint8_t* lab = nullptr;
@@ -812,8 +832,8 @@ int setup_tiles(Ctx& c) {
 
 int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) {
   if (!p || !raw || !out) return fail(PIKS_E_ARG, "null argument");
-  if (p->precision != PIKS_PREC_FP64)
-    return fail(PIKS_E_ARG, "precision mode " + std::to_string(p->precision) + " is not available");
+  if (p->precision != PIKS_PREC_FP64 && p->precision != PIKS_PREC_FP32_REFINED)
+    return fail(PIKS_E_ARG, "unknown precision mode " + std::to_string(p->precision));
   if (!(p->kn > 0.0)) return fail(PIKS_E_ARG, "Kn must be positive");
   if (p->chk_converge_step < 1) return fail(PIKS_E_ARG, "chkConvergeStep must be >= 1");
   std::unique_ptr<Ctx, void (*)(Ctx*)> c(new Ctx(), free_ctx);
@@ -834,25 +854,35 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
   const size_t nf = (size_t)g.Ntotal * 8 * c->d.Nc8;
+  c->n_f = nf;
+  c->n_fw = (size_t)std::max(g.ncorner, 1) * 8 * c->d.Nc8;
+  c->n_buf = (size_t)g.Ny * g.Nz * 8 * c->d.Nc8;
+  c->n_snap = std::max<size_t>(g.snap_cell.size(), 1) * 8 * c->d.Nc8;
+  const bool refined = p->precision == PIKS_PREC_FP32_REFINED;
   size_t free_b = 0, total_b = 0;
   HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-  const size_t need = nf * sizeof(double) + (size_t)g.Ntotal * (32 * 8 * c->nchunk + 4 * 8 + 16) +
-                      g.sweep_cells.size() * 4;
+  const size_t need = nf * (sizeof(double) + (refined ? sizeof(float) : 0)) +
+                      (size_t)g.Ntotal * (32 * 8 * c->nchunk + 4 * 8 + 16) + g.sweep_cells.size() * 6;
   if (need > free_b)
     return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
                                 std::to_string(free_b >> 20) + " MiB free");
   int rc = 0;
-  if ((rc = dalloc(*c, &c->f, nf))) return rc;
-  if ((rc = dalloc(*c, &c->fw, (size_t)std::max(g.ncorner, 1) * 8 * c->d.Nc8))) return rc;
+  if ((rc = dalloc(*c, &c->s64.f, nf)) || (rc = dalloc(*c, &c->s64.fw, c->n_fw)) ||
+      (rc = dalloc(*c, &c->s64.buf_e, c->n_buf)) || (rc = dalloc(*c, &c->s64.buf_w, c->n_buf)) ||
+      (rc = dalloc(*c, &c->s64.snap, c->n_snap)))
+    return rc;
+  if (refined) {
+    if ((rc = dalloc(*c, &c->s32.f, nf)) || (rc = dalloc(*c, &c->s32.fw, c->n_fw)) ||
+        (rc = dalloc(*c, &c->s32.buf_e, c->n_buf)) || (rc = dalloc(*c, &c->s32.buf_w, c->n_buf)) ||
+        (rc = dalloc(*c, &c->s32.snap, c->n_snap)))
+      return rc;
+    c->phase = PIKS_PREC_FP32_REFINED;
+  }
   if ((rc = dalloc(*c, &c->rho, (size_t)g.Ntotal))) return rc;
   if ((rc = dalloc(*c, &c->ux, (size_t)g.Ntotal))) return rc;
   if ((rc = dalloc(*c, &c->uy, (size_t)g.Ntotal))) return rc;
   if ((rc = dalloc(*c, &c->uz, (size_t)g.Ntotal))) return rc;
   if ((rc = dalloc(*c, &c->part, (size_t)g.Ntotal * 32 * c->nchunk))) return rc;
-  const size_t nb = (size_t)g.Ny * g.Nz * 8 * c->d.Nc8;
-  if ((rc = dalloc(*c, &c->buf_e, nb))) return rc;
-  if ((rc = dalloc(*c, &c->buf_w, nb))) return rc;
-  if ((rc = dalloc(*c, &c->snap, std::max<size_t>(g.snap_cell.size(), 1) * 8 * c->d.Nc8))) return rc;
   if ((rc = dalloc(*c, &c->plane, (size_t)g.Ny * g.Nz))) return rc;
   if ((rc = upload(*c, &c->dcx, c->cx)) || (rc = upload(*c, &c->dcy, c->cy)) ||
       (rc = upload(*c, &c->dcz, c->cz)) || (rc = upload(*c, &c->dw, c->w)))
@@ -878,30 +908,31 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
 }
 
 template <typename T>
-WallArgs<T> wall_args(Ctx& c, const int32_t* list, int n) {
+WallArgs<T> wall_args(Ctx& c, State<T>& st, const int32_t* list, int n) {
   WallArgs<T> a{};
-  a.f = c.f; a.fw = c.fw; a.list = list; a.n = n;
+  a.f = st.f; a.fw = st.fw; a.list = list; a.n = n;
   a.wall_cell = c.wall_cell; a.wall_label = c.wall_label; a.e1 = c.e1; a.e2 = c.e2;
   a.wall_corner = c.wall_corner;
   a.cx = c.dcx; a.cy = c.dcy; a.cz = c.dcz; a.w = c.dw;
   a.accom = c.prm.accom; a.diff_flux = c.diff_flux; a.inv_diff_flux = 1.0 / c.diff_flux;
   a.d = c.d; a.exact = c.exact;
-  a.snap_idx = c.coupled_snap; a.snap = c.snap;
+  a.snap_idx = c.coupled_snap; a.snap = st.snap;
   return a;
 }
 
-int iterate(Ctx& c, int nsteps) {
+template <typename T>
+int iterate_t(Ctx& c, State<T>& st, int nsteps) {
   HIP_TRY(hipSetDevice(c.device));
   const Geometry& g = c.g;
   const Dims d = c.d;
-  SweepArgs<double> sa{};
-  sa.f = c.f; sa.lab = c.lab; sa.code = c.code;
+  SweepArgs<T> sa{};
+  sa.f = st.f; sa.lab = c.lab; sa.code = c.code;
   sa.cells = c.sweep_cells; sa.local = c.sweep_local; sa.off = c.sweep_off;
   sa.NTY = g.NTY; sa.NTZ = g.NTZ; sa.nh = g.nh; sa.nchunk = c.nchunk;
   sa.NT = (int64_t)g.NTX * g.NTY * g.NTZ;
   sa.rho = c.rho; sa.ux = c.ux; sa.uy = c.uy; sa.uz = c.uz;
   sa.part = c.exact ? nullptr : c.part;
-  sa.wid = c.wid; sa.views = c.views; sa.wall_corner = c.wall_corner; sa.fw = c.fw;
+  sa.wid = c.wid; sa.views = c.views; sa.wall_corner = c.wall_corner; sa.fw = st.fw;
   sa.cx = c.dcx; sa.cy = c.dcy; sa.cz = c.dcz; sa.w = c.dw;
   sa.mu = c.mu;
   sa.c0 = 1.5 / g.ds; sa.c1 = 2.0 / g.ds; sa.c2 = -0.5 / g.ds; sa.d1 = 1.0 / g.ds;
@@ -919,42 +950,42 @@ int iterate(Ctx& c, int nsteps) {
     for (int dg = 0; dg < c.ndiag; ++dg) {
       const int nt = c.diag_off[dg + 1] - c.diag_off[dg];
       sa.tiles = c.tiles + c.diag_off[dg];
-      hipLaunchKernelGGL(k_sweep<double>, dim3(nt * 8 * c.nchunk), dim3(kSweepThreads), 0, c.stream, sa);
+      hipLaunchKernelGGL(k_sweep<T>, dim3(nt * 8 * c.nchunk), dim3(kSweepThreads), 0, c.stream, sa);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
-      hipLaunchKernelGGL(k_commit<double>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb)),
-                         dim3(wpb * kWave), 0, c.stream, c.f, c.fw, c.commit_walls,
+      hipLaunchKernelGGL(k_commit<T>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb)),
+                         dim3(wpb * kWave), 0, c.stream, st.f, (const T*)st.fw, c.commit_walls,
                          (int)g.commit_walls.size(), c.wall_cell, c.wall_corner, c.views, d);
-    hipLaunchKernelGGL(k_halo<double>, dim3(grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256)), dim3(256),
-                       0, c.stream, c.f, c.buf_e, c.buf_w, d);
+    hipLaunchKernelGGL(k_halo<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256)), dim3(256),
+                       0, c.stream, st.f, st.buf_e, st.buf_w, d);
     if (!g.snap_cell.empty())
-      hipLaunchKernelGGL(k_snapshot<double>, dim3(grid_for((int64_t)g.snap_cell.size() * 8 * d.Nc8, 256)),
-                         dim3(256), 0, c.stream, (const double*)c.f, c.snap, c.snap_cell,
+      hipLaunchKernelGGL(k_snapshot<T>, dim3(grid_for((int64_t)g.snap_cell.size() * 8 * d.Nc8, 256)),
+                         dim3(256), 0, c.stream, (const T*)st.f, st.snap, c.snap_cell,
                          (int)g.snap_cell.size(), d);
     if (!g.uncoupled.empty()) {
-      WallArgs<double> wa = wall_args<double>(c, c.uncoupled, (int)g.uncoupled.size());
-      hipLaunchKernelGGL(k_walls<double>, dim3(grid_for((int64_t)g.uncoupled.size(), wpb)),
+      WallArgs<T> wa = wall_args<T>(c, st, c.uncoupled, (int)g.uncoupled.size());
+      hipLaunchKernelGGL(k_walls<T>, dim3(grid_for((int64_t)g.uncoupled.size(), wpb)),
                          dim3(wpb * kWave), 0, c.stream, wa);
     }
     for (size_t lv = 0; lv + 1 < g.coupled_level_off.size(); ++lv) {
       const int b = g.coupled_level_off[lv], n = g.coupled_level_off[lv + 1] - b;
-      WallArgs<double> wa = wall_args<double>(c, c.coupled + b, n);
+      WallArgs<T> wa = wall_args<T>(c, st, c.coupled + b, n);
       wa.snap_idx = c.coupled_snap + (size_t)b * 6;
-      hipLaunchKernelGGL(k_walls_coupled<double>, dim3(grid_for(n, wpb)), dim3(wpb * kWave), 0,
+      hipLaunchKernelGGL(k_walls_coupled<T>, dim3(grid_for(n, wpb)), dim3(wpb * kWave), 0,
                          c.stream, wa);
     }
-    hipLaunchKernelGGL(k_inout<double>, dim3(grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256)), dim3(256),
-                       0, c.stream, c.f, (const double*)c.dw, c.prm.press_drop, d);
-    hipLaunchKernelGGL(k_sym_y<double>, dim3(grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256)), dim3(256), 0,
-                       c.stream, c.f, d);
-    hipLaunchKernelGGL(k_sym_z<double>, dim3(grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256)), dim3(256), 0,
-                       c.stream, c.f, d);
+    hipLaunchKernelGGL(k_inout<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256)), dim3(256),
+                       0, c.stream, st.f, (const double*)c.dw, c.prm.press_drop, d);
+    hipLaunchKernelGGL(k_sym_y<T>, dim3(grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256)), dim3(256), 0,
+                       c.stream, st.f, d);
+    hipLaunchKernelGGL(k_sym_z<T>, dim3(grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256)), dim3(256), 0,
+                       c.stream, st.f, d);
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 2], c.stream));
     if (c.exact) {
       const int64_t n = (int64_t)g.fluid_interior.size() + (int64_t)g.fluid_boundary.size();
-      hipLaunchKernelGGL(k_moments_serial<double>, dim3(grid_for(n, 64)), dim3(64), 0, c.stream,
-                         (const double*)c.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, c.rho, c.ux,
+      hipLaunchKernelGGL(k_moments_serial<T>, dim3(grid_for(n, 64)), dim3(64), 0, c.stream,
+                         (const T*)st.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, c.rho, c.ux,
                          c.uy, c.uz);
     } else {
       const int64_t ni = (int64_t)g.fluid_interior.size(), nbd = (int64_t)g.fluid_boundary.size();
@@ -962,8 +993,8 @@ int iterate(Ctx& c, int nsteps) {
         hipLaunchKernelGGL(k_moments_partials, dim3(grid_for(ni, 256)), dim3(256), 0, c.stream,
                            (const double*)c.part, c.nchunk, c.fluid_int, ni, c.rho, c.ux, c.uy, c.uz);
       if (nbd > 0)
-        hipLaunchKernelGGL(k_moments_direct<double>, dim3((unsigned)((nbd + wpb - 1) / wpb)),
-                           dim3(wpb * kWave), 0, c.stream, (const double*)c.f, c.fluid_bnd, nbd,
+        hipLaunchKernelGGL(k_moments_direct<T>, dim3((unsigned)((nbd + wpb - 1) / wpb)),
+                           dim3(wpb * kWave), 0, c.stream, (const T*)st.f, c.fluid_bnd, nbd,
                            c.dcx, c.dcy, c.dcz, d, c.rho, c.ux, c.uy, c.uz);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 3], c.stream));
@@ -982,6 +1013,35 @@ int iterate(Ctx& c, int nsteps) {
     HIP_TRY(hipEventElapsedTime(&all, c.ev[0], c.ev[4 * (timed - 1) + 3]));
     c.last_ms[0] = acc[0]; c.last_ms[1] = acc[1]; c.last_ms[2] = acc[2]; c.last_ms[3] = all;
   }
+  return 0;
+}
+
+int iterate(Ctx& c, int nsteps) {
+  if (c.phase == PIKS_PREC_FP32_REFINED) return iterate_t<float>(c, c.s32, nsteps);
+  return iterate_t<double>(c, c.s64, nsteps);
+}
+
+// Move the carried state between precisions (the moments are FP64 in both).
+int set_phase(Ctx& c, int phase) {
+  if (phase == c.phase) return 0;
+  if (phase == PIKS_PREC_FP32_REFINED && !c.s32.f)
+    return fail(PIKS_E_STATE, "context was created without FP32 storage");
+  HIP_TRY(hipSetDevice(c.device));
+  const int blocks = 4096;
+  if (phase == PIKS_PREC_FP64) {
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.f, (const float*)c.s32.f, c.n_f);
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.fw, (const float*)c.s32.fw, c.n_fw);
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_e, (const float*)c.s32.buf_e, c.n_buf);
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_w, (const float*)c.s32.buf_w, c.n_buf);
+  } else {
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.f, (const double*)c.s64.f, c.n_f);
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.fw, (const double*)c.s64.fw, c.n_fw);
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_e, (const double*)c.s64.buf_e, c.n_buf);
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_w, (const double*)c.s64.buf_w, c.n_buf);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  c.phase = phase;
   return 0;
 }
 
@@ -1033,6 +1093,13 @@ int piks_iterate(void* ctx, int32_t nsteps) {
   return iterate(*static_cast<Ctx*>(ctx), nsteps);
 }
 
+int piks_set_precision(void* ctx, int32_t precision) {
+  if (!ctx) return fail(PIKS_E_ARG, "bad context");
+  if (precision != PIKS_PREC_FP64 && precision != PIKS_PREC_FP32_REFINED)
+    return fail(PIKS_E_ARG, "unknown precision mode " + std::to_string(precision));
+  return set_phase(*static_cast<Ctx*>(ctx), precision);
+}
+
 int piks_mass_flow(void* ctx, double* m) {
   if (!ctx || !m) return fail(PIKS_E_ARG, "null argument");
   return mass_flow(*static_cast<Ctx*>(ctx), m);
@@ -1072,8 +1139,15 @@ int piks_get_f(void* ctx, double* f) {
   HIP_TRY(hipSetDevice(c.device));
   const size_t nc8 = c.d.Nc8, nt = (size_t)c.g.Ntotal;
   std::vector<double> h(nt * 8 * nc8);
-  HIP_TRY(hipMemcpyAsync(h.data(), c.f, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_TRY(hipStreamSynchronize(c.stream));
+  if (c.phase == PIKS_PREC_FP32_REFINED) {
+    std::vector<float> hf(h.size());
+    HIP_TRY(hipMemcpyAsync(hf.data(), c.s32.f, hf.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    for (size_t q = 0; q < h.size(); ++q) h[q] = hf[q];
+  } else {
+    HIP_TRY(hipMemcpyAsync(h.data(), c.s64.f, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+  }
   // device [o][cell][l] -> Fortran f(cell, l, o)
   for (size_t o = 0; o < 8; ++o)
     for (size_t cell = 0; cell < nt; ++cell)
@@ -1088,8 +1162,15 @@ int piks_get_fw(void* ctx, double* fw) {
   const size_t nc8 = c.d.Nc8, ncor = (size_t)c.g.ncorner;
   if (ncor == 0) return 0;
   std::vector<double> h(ncor * 8 * nc8);
-  HIP_TRY(hipMemcpyAsync(h.data(), c.fw, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_TRY(hipStreamSynchronize(c.stream));
+  if (c.phase == PIKS_PREC_FP32_REFINED) {
+    std::vector<float> hf(h.size());
+    HIP_TRY(hipMemcpyAsync(hf.data(), c.s32.fw, hf.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    for (size_t q = 0; q < h.size(); ++q) h[q] = hf[q];
+  } else {
+    HIP_TRY(hipMemcpyAsync(h.data(), c.s64.fw, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+  }
   for (size_t o = 0; o < 8; ++o)
     for (size_t q = 0; q < ncor; ++q)
       for (size_t l = 0; l < nc8; ++l) fw[(o * nc8 + l) * ncor + q] = h[(q * 8 + o) * nc8 + l];
@@ -1120,7 +1201,7 @@ int piks_get_info(void* ctx, piks_info* info) {
   info->mass = c.mass;
   info->device_bytes = c.bytes;
   info->coupled_walls = (int32_t)g.coupled.size();
-  info->precision = c.prm.precision;
+  info->precision = c.phase;
   return 0;
 }
 

@@ -12,7 +12,11 @@
 ! legacy VTK in the formats of src/output.F90:166-360.
 !
 ! One optional namelist is added, ignored by the reference:
-!     &gpuNml precision = 'fp64' | 'fp32', device = 0, exact_quadrature = F /
+!     &gpuNml precision = 'fp64' | 'fp32', device = 0, exact_quadrature = F,
+!             refineSwitch = 1.0d-6 /
+! With precision = 'fp32' the iterations run in FP32 until the check's error
+! is below refineSwitch or has stopped falling for three checks; the rest run
+! in FP64, so the converged permeability is the FP64 solve's.
 program piks3d_gpu
     use iso_c_binding
     use piks_c
@@ -32,12 +36,13 @@ program piks3d_gpu
     character(len=8) :: precision
     integer :: device
     logical :: exact_quadrature
+    double precision :: refineSwitch
     namelist /physicalNml/ imageFileName, Nx, Ny, Nz, wallExtOrder, fluidlayer, Ref_L
     namelist /velocityNml/ Nc_fundamental, halfRange
     namelist /mpiNml/ mpi_xdim, mpi_ydim, mpi_zdim, block_repx, block_repy, block_repz
     namelist /solverNml/ maxStep, chkConvergeStep, saveStep, eps, saveLast, saveFormat
     namelist /flowNml/ Kn, pressDrop, accom
-    namelist /gpuNml/ precision, device, exact_quadrature
+    namelist /gpuNml/ precision, device, exact_quadrature, refineSwitch
 
     integer, parameter :: PARAFILE = 10, IMAGEFILE = 20
     type(piks_params), target :: prm
@@ -47,7 +52,9 @@ program piks3d_gpu
     double precision, allocatable, target :: rho(:), ux(:), uy(:), uz(:)
     integer(c_int32_t), allocatable, target :: labels(:)
     integer :: ios, rc, iStep, nstep, nbytes, next_stop
-    double precision :: error, mass, permeability, out(3)
+    double precision :: error, mass, permeability, out(3), best
+    logical :: refined
+    integer :: stalled
     integer(8) :: t0, t1, trate
 
     ! ------------------------------------------------------------ para.in
@@ -61,6 +68,7 @@ program piks3d_gpu
     precision = 'fp64'
     device = 0
     exact_quadrature = .false.
+    refineSwitch = 1.0d-6
 
     open(unit=PARAFILE, file='para.in', status='old', iostat=ios)
     if (ios /= 0) then
@@ -133,6 +141,9 @@ program piks3d_gpu
     error = 1.d0
     permeability = 0.d0
     mass = 1.d0
+    refined = info%precision == PIKS_PREC_FP32_REFINED
+    best = huge(1.d0)
+    stalled = 0
     call system_clock(t0, trate)
     iStep = 0
     do while (iStep < maxStep)
@@ -145,6 +156,22 @@ program piks3d_gpu
         iStep = next_stop
         if (mod(iStep, chkConvergeStep) == 0) call chk_converge()
         if (mod(iStep, saveStep) == 0) call save_fields()
+        if (refined) then
+            if (mod(iStep, chkConvergeStep) /= 0) cycle
+            if (error < best) then
+                best = error
+                stalled = 0
+            else
+                stalled = stalled + 1
+            endif
+            if (error <= max(refineSwitch, eps) .or. stalled >= 3) then
+                rc = piks_set_precision(ctx, PIKS_PREC_FP64)
+                call check_rc(rc)
+                refined = .false.
+                error = 1.d0
+            endif
+            cycle
+        endif
         if (error <= eps) then
             if (iStep < chkConvergeStep) call chk_converge()
             exit
@@ -366,6 +393,7 @@ contains
         print*, "pressDrop = ", pressDrop
         print*, "accom = ", accom
         print*, "precision = ", trim(precision)
+        if (trim(precision) == 'fp32') print*, "refineSwitch = ", refineSwitch
         print*, "======================================"
     end subroutine print_params
 

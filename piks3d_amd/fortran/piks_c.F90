@@ -57,6 +57,12 @@ module piks_c
             integer(c_int32_t), value :: nsteps
         end function piks_iterate
 
+        integer(c_int) function piks_set_precision(ctx, precision) bind(C, name='piks_set_precision')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: precision
+        end function piks_set_precision
+
         integer(c_int) function piks_chk_converge(ctx, out) bind(C, name='piks_chk_converge')
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: ctx

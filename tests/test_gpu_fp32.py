@@ -1,0 +1,72 @@
+"""FP32 storage with FP64 refinement against the plain FP64 solve.
+
+The refined context keeps f, fw and the exchange planes in FP32 while the
+moments, the wall densities and the mass-flow check stay FP64; the solve runs
+FP32 iterations until the check stalls and finishes in FP64.  The stated
+bound: the converged apparent permeability within 1e-8 relative of the plain
+FP64 solve's at the same eps.
+"""
+import numpy as np
+import pytest
+
+from piks3d_amd import PiksError, Solver, images
+from piks3d_amd.params import Params
+
+pytestmark = pytest.mark.gpu
+
+
+def _pack(n=40, kn=1.0, seed=5):
+    img = images.sphere_pack(n, 0.40, 3.0, 6.0, seed=seed)
+    p = Params(nx=n, ny=n, nz=n, fluid_layer=2, wall_ext_order=2, nc_fundamental=4,
+               half_range=True, chk_converge_step=10, max_step=20000, eps=1e-10,
+               kn=kn, press_drop=1.0, accom=1.0)
+    return p, img
+
+
+def test_fp32_iterations_track_fp64_to_single_precision():
+    p, img = _pack()
+    with Solver(p, img, precision="fp64") as a, Solver(p, img, precision="fp32") as b:
+        a.iterate(5)
+        b.iterate(5)
+        ra, rb = a.macro(), b.macro()
+        fa, fb = a.f(), b.f()
+    for x, y in zip(ra, rb):
+        scale = np.abs(x).max()
+        assert scale > 0 and np.abs(x - y).max() <= 1e-5 * scale
+    assert np.abs(fa - fb).max() <= 1e-5 * np.abs(fa).max()
+    # FP32 rounding really happened: the two paths are not bit-identical
+    assert not np.array_equal(fa, fb)
+
+
+@pytest.mark.parametrize("kn", [0.1, 1.0])
+def test_refined_solve_reaches_the_fp64_permeability(kn):
+    p, img = _pack(kn=kn)
+    with Solver(p, img, precision="fp64") as a:
+        r64 = a.solve()
+    with Solver(p, img, precision="fp32") as b:
+        r32 = b.solve()
+        assert b.info.precision == 0          # ended in FP64
+    assert r64["error"] <= p.eps and r32["error"] <= p.eps
+    assert 0 < r32["steps32"] < r32["steps"]
+    k64, k32 = r64["permeability"], r32["permeability"]
+    assert abs(k32 - k64) <= 1e-8 * abs(k64), (k32, k64, r32["steps32"], r32["steps"], r64["steps"])
+
+
+def test_precision_switch_rules():
+    p, img = _pack(n=24)
+    with Solver(p, img, precision="fp64") as a:
+        with pytest.raises(PiksError) as e:
+            a.set_precision("fp32")
+        assert e.value.code == -7
+        with pytest.raises(PiksError) as e:
+            a.set_precision(5)
+        assert e.value.code == -1
+        a.set_precision("fp64")                  # no-op
+    with Solver(p, img, precision="fp32") as b:
+        assert b.info.precision == 1
+        b.iterate(3)
+        b.set_precision("fp64")
+        f64 = b.f()
+        b.set_precision("fp32")                  # back down: values rounded to FP32
+        f32 = b.f()
+        assert np.array_equal(f32, f64.astype(np.float32).astype(np.float64))
