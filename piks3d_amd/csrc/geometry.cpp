@@ -328,64 +328,49 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
       }
     g.coupled_level_off.push_back((int32_t)g.coupled.size());
   }
-  build_sweep_schedule(g, g.TX, g.TY, g.TZ);
+  if (g.Ntotal >= ((int64_t)1 << 31)) {
+    err = "domain of " + std::to_string(g.Ntotal) + " cells (with ghosts) is too large";
+    return PIKS_E_ARG;
+  }
+  build_sweep_schedule(g);
   return PIKS_OK;
 }
 
-void build_sweep_schedule(Geometry& g, int tx, int ty, int tz) {
-  g.TX = tx; g.TY = ty; g.TZ = tz;
-  g.NTX = (g.Nx + tx - 1) / tx;
-  g.NTY = (g.Ny + ty - 1) / ty;
-  g.NTZ = (g.Nz + tz - 1) / tz;
-  g.nh = tx + ty + tz - 2;
-  const int64_t NT = (int64_t)g.NTX * g.NTY * g.NTZ;
-  const int nh1 = g.nh + 1;
-  g.sweep_off.assign((size_t)(8 * NT * nh1), 0);
-  std::vector<int64_t> key;     // bucket of each swept cell, per octant
-  std::vector<int32_t> cells;
-  std::vector<uint16_t> local;
-  g.sweep_cells.clear();
-  g.sweep_local.clear();
-  for (int o = 0; o < 8; ++o) {
-    const int sx = osx(o), sy = osy(o), sz = osz(o);
-    std::vector<int32_t> count((size_t)(NT * g.nh), 0);
-    key.clear();
-    cells.clear();
-    local.clear();
-    for (int k = 1; k <= g.Nz; ++k)
-      for (int j = 1; j <= g.Ny; ++j)
-        for (int i = 1; i <= g.Nx; ++i) {
-          int64_t c = g.cell(i, j, k);
-          if (g.lab[(size_t)c] != FLUID) continue;
-          const int oi = sx > 0 ? i - 1 : g.Nx - i;
-          const int oj = sy > 0 ? j - 1 : g.Ny - j;
-          const int ok = sz > 0 ? k - 1 : g.Nz - k;
-          if (oi < 1 || oj < 1 || ok < 1) continue;  // the octant's inflow layer is not swept
-          const int64_t t = ((int64_t)(oi / tx) * g.NTY + oj / ty) * g.NTZ + ok / tz;
-          const int h = oi % tx + oj % ty + ok % tz;
-          const int64_t b = t * g.nh + h;
-          key.push_back(b);
-          cells.push_back((int32_t)c);
-          local.push_back((uint16_t)(oi % tx | (oj % ty) << 5 | (ok % tz) << 10));
-          ++count[(size_t)b];
-        }
-    // counting sort by (tile, hyperplane); within a bucket the raster order is kept
-    std::vector<int64_t> start((size_t)(NT * g.nh) + 1, 0);
-    for (size_t b = 0; b < count.size(); ++b) start[b + 1] = start[b] + count[b];
-    const int64_t base = (int64_t)g.sweep_cells.size();
-    g.sweep_cells.resize((size_t)(base + (int64_t)cells.size()));
-    g.sweep_local.resize(g.sweep_cells.size());
-    std::vector<int64_t> pos(start.begin(), start.end() - 1);
-    for (size_t q = 0; q < cells.size(); ++q) {
-      const size_t at = (size_t)(base + pos[(size_t)key[q]]++);
-      g.sweep_cells[at] = cells[q];
-      g.sweep_local[at] = local[q];
+void build_sweep_schedule(Geometry& g) {
+  g.nhp = (g.Nx - 1) + (g.Ny - 1) + (g.Nz - 1) + 1;
+  const int64_t nb = (int64_t)g.nhp * 8;
+  std::vector<int64_t> count((size_t)nb + 1, 0);
+  auto oriented = [&](int o, int i, int j, int k, int* oi, int* oj, int* ok) {
+    *oi = osx(o) > 0 ? i - 1 : g.Nx - i;
+    *oj = osy(o) > 0 ? j - 1 : g.Ny - j;
+    *ok = osz(o) > 0 ? k - 1 : g.Nz - k;
+    return *oi >= 1 && *oj >= 1 && *ok >= 1;  // the octant's inflow layer is not swept
+  };
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int64_t> pos;
+    if (pass == 1) {
+      g.hp_off.assign((size_t)g.nhp + 1, 0);
+      for (int64_t b = 0; b < nb; ++b) count[(size_t)b + 1] += count[(size_t)b];
+      for (int s = 0; s <= g.nhp; ++s) g.hp_off[(size_t)s] = count[(size_t)s * 8];
+      g.hp_task.assign((size_t)count[(size_t)nb], 0);
+      pos.assign(count.begin(), count.end() - 1);
     }
-    for (int64_t t = 0; t < NT; ++t) {
-      int32_t* off = &g.sweep_off[(size_t)(((int64_t)o * NT + t) * nh1)];
-      for (int h = 0; h < g.nh; ++h) off[h] = (int32_t)(base + start[(size_t)(t * g.nh + h)]);
-      off[g.nh] = (int32_t)(base + start[(size_t)((t + 1) * g.nh)]);
-    }
+    for (int o = 0; o < 8; ++o)
+      for (int k = 1; k <= g.Nz; ++k)
+        for (int j = 1; j <= g.Ny; ++j)
+          for (int i = 1; i <= g.Nx; ++i) {
+            const int64_t c = g.cell(i, j, k);
+            int oi, oj, ok;
+            if (g.lab[(size_t)c] != FLUID || !oriented(o, i, j, k, &oi, &oj, &ok)) continue;
+            const int64_t b = (int64_t)(oi + oj + ok) * 8 + o;
+            if (pass == 0)
+              ++count[(size_t)b + 1];
+            else {
+              const uint32_t cc = g.code[(size_t)c];
+              const uint64_t flags = ((cc >> (3 * o)) & 7u) | (((cc >> (24 + o)) & 1u) << 3);
+              g.hp_task[(size_t)pos[(size_t)b]++] = (uint64_t)c | (uint64_t)o << 32 | flags << 35;
+            }
+          }
   }
 }
 

@@ -54,15 +54,18 @@ struct Geometry {
   std::vector<int32_t> commit_walls;      // walls with a switch in some octant
   int64_t nfluid = 0;
 
-  // Sweep schedule.  Each octant tiles the interior in its own oriented frame
-  // (oriented x = i-1 for cx > 0, Nx-i for cx < 0; likewise y, z) with tiles
-  // of TX x TY x TZ cells.  sweep_cells holds, per octant and tile, the cells
-  // the octant updates, ordered by in-tile hyperplane a+b+c;
-  // sweep_off[((o*NT + t)*(nh+1)) + h] is where hyperplane h of tile t starts.
-  int TX = 8, TY = 8, TZ = 8, NTX = 0, NTY = 0, NTZ = 0, nh = 0;
-  std::vector<int32_t> sweep_cells;
-  std::vector<uint16_t> sweep_local;   // in-tile oriented (a | b << 5 | c << 10) of each entry
-  std::vector<int32_t> sweep_off;
+  // Sweep schedule.  Octant o updates its swept cells (fluid cells past the
+  // octant's inflow layer) in order of the oriented hyperplane
+  // s = oi + oj + ok, where oi = i-1 when the octant's cx > 0 and Nx-i when
+  // cx < 0 (likewise oj, ok): a cell's upwind neighbours lie on s-1 and s-2,
+  // so one hyperplane of all eight octants can be updated at once.
+  // hp_task[hp_off[s] .. hp_off[s+1]) lists the tasks of hyperplane s,
+  // octant-major, raster order within an octant.  A task word holds the cell
+  // (bits 0-31), the octant (32-34), the octant's first-order flags for x, y,
+  // z (35-37) and its "reads a switched wall" flag (38).
+  int nhp = 0;
+  std::vector<uint64_t> hp_task;
+  std::vector<int64_t> hp_off;
 
   int64_t cell(int i, int j, int k) const {
     return (int64_t)(i + 1) + (int64_t)(j + 1) * Nxt + (int64_t)(k + 1) * Nxyt;
@@ -73,7 +76,7 @@ struct Geometry {
 int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geometry& g,
                    std::string& err);
 
-// Fill the sweep schedule for tiles of tx x ty x tz cells (each <= 32).
-void build_sweep_schedule(Geometry& g, int tx, int ty, int tz);
+// Fill the hyperplane sweep schedule (hp_task, hp_off).
+void build_sweep_schedule(Geometry& g);
 
 }  // namespace piks

@@ -2,9 +2,9 @@
 // voxel image, behind the C ABI in include/piks3d.h.
 //
 // One source iteration of the reference (src/solver.F90:111-1940) becomes:
-//   sweep      8 octants at once; hyperplane wavefront over tiles, one launch
-//              per tile diagonal; a 64-lane wave updates all velocities of one
-//              cell (f loads coalesced along the velocity index); partial
+//   sweep      8 octants at once; one launch per oriented hyperplane
+//              i+j+k; a 64-lane wave updates all velocities of one (cell,
+//              octant) (f loads coalesced along the velocity index); partial
 //              moments of each (cell, octant) reduced in registers
 //   commit     the reference's in-sweep wall switches, applied after the sweep
 //              (readers were redirected to the switched sources during it)
@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -76,7 +77,6 @@ struct Dims {
 template <typename T>
 struct SweepArgs {
   T* f;
-  const uint32_t* code;
   const double* rho;
   const double* ux;
   const double* uy;
@@ -93,20 +93,12 @@ struct SweepArgs {
   const double* w;
   double mu, c0, c1, c2, d1;  // 1.5/ds, 2/ds, -0.5/ds, 1/ds
   Dims d;
-  const int4* tiles;       // oriented tile coordinates of this diagonal
-  const int32_t* cells;    // swept cells per (octant, tile), by in-tile hyperplane
-  const uint16_t* local;   // their in-tile oriented coordinates a | b << 5 | c << 10
-  const int32_t* off;      // hyperplane offsets into `cells`
-  int NTY, NTZ, nh, nchunk;
-  int64_t NT;
+  const uint64_t* tasks;   // this hyperplane's task words (cell, octant, flags)
+  int64_t ntask;
+  int nchunk;
+  bool xcd_remap;
 };
 
-// Sweep tiles are kTile^3 cells of one octant's oriented grid.
-constexpr int kTile = 8;
-constexpr int kTileH = 3 * kTile - 2;            // in-tile hyperplanes a+b+c
-constexpr int kTileHMax = 48;                    // most cells on one hyperplane of an 8^3 tile
-constexpr int kTileCells = kTile * kTile * kTile;
-constexpr int kRound = 2;                        // cells a wave updates together
 constexpr int kSweepThreads = 256;
 
 // Value a cell reading neighbour n along `axis` (1 = y, 2 = z) sees in octant
@@ -140,182 +132,120 @@ __device__ __forceinline__ double wave_sum4(double a, double b, double c, double
   return z;
 }
 
-// Everything one update needs that does not depend on this tile's progress:
-// the cell, its flags, its own previous f, and the upstream neighbours that
-// are not updated in this tile (other tiles, walls, inflow layers).
-template <typename T>
-struct Pending {
-  int e;             // entry in the tile's list, -1 if this slot is idle
-  int32_t cell;
-  uint32_t code;
-  double r, u, v, w; // moments of the previous iteration at the cell
-  T fp;
-  T nb[6];           // x1 x2 y1 y2 z1 z2 when read from memory
-  int16_t ring[6];   // LDS ring slot of the neighbour, -1 when nb[] holds it
-};
+// Workgroups are dispatched round-robin over the eight XCDs (block b runs on
+// XCD b % 8).  Renumber them so that XCD x runs the x-th contiguous eighth of
+// the launch's tasks: consecutive tasks of one octant share upwind
+// neighbours, which then stay in that XCD's L2.  Launch grids are multiples
+// of 8 blocks.
+__device__ __forceinline__ int64_t xcd_block(bool remap) {
+  if (!remap) return blockIdx.x;
+  const int per = gridDim.x / 8;
+  return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
 
-// One octant's sweep over one tile for one chunk of 64 velocities.  Cells of
-// in-tile hyperplane h depend only on hyperplanes h-1, h-2 of this tile and on
-// upstream tiles (earlier launches); a barrier per hyperplane orders the tile
-// and the new values of the last three hyperplanes stay in LDS.
+// One hyperplane of the eight octant sweeps.  A wave updates kSweepBatch
+// (B) (cell, octant, chunk of 64 velocities) tasks: their upwind neighbours lie
+// on the two previous hyperplanes (earlier launches), so no wave waits on
+// another; the loads of all its tasks are issued before any is used.
 // Each update is src/solver.F90:206-214 (and analogues) with the same
 // operations in the same order (contraction off): signs enter as exact
 // negations, and a first-order axis contributes (c*0)*f like the reference.
-template <typename T>
+template <typename T, int kSweepBatch>
 __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<T> a) {
-  __shared__ T ring[3 * kTileHMax * kWave];
-  __shared__ int32_t s_cell[kTileCells];
-  __shared__ uint32_t s_code[kTileCells];
-  __shared__ uint16_t s_local[kTileCells];
-  __shared__ int16_t s_pos[kTileCells];
-  __shared__ int32_t s_hoff[kTileH + 1];
-
   const int nch = a.nchunk;
-  const int ch = blockIdx.x % nch;
-  const int o = (blockIdx.x / nch) & 7;
-  const int4 tile = a.tiles[blockIdx.x / (nch * 8)];
+  // the wave index is uniform over the wave: keep it (and everything derived
+  // from it: tasks, cells, octants, addresses' uniform parts) in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t wv = xcd_block(a.xcd_remap) * (kSweepThreads / kWave) + wave;
+  const int64_t nwork = a.ntask * nch;
+  if (wv * kSweepBatch >= nwork) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int nwaves = blockDim.x / kWave;
   const int Nc8 = a.d.Nc8;
-  const int l = ch * kWave + lane;
-  const bool lact = l < Nc8;
-  const int ll = lact ? l : 0;
-  const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
-  const int sx = osx(o), sy = osy(o), sz = osz(o);
-  const int64_t step[3] = {(int64_t)sx, (int64_t)sy * a.d.Nxt, (int64_t)sz * a.d.Nxyt};
-  T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8 + ll;
-  const int64_t tid = ((int64_t)tile.x * a.NTY + tile.y) * a.NTZ + tile.z;
-  const int32_t* off = a.off + ((int64_t)o * a.NT + tid) * (kTileH + 1);
-  const int base = off[0];
+  const int64_t Nxt = a.d.Nxt, Nxyt = a.d.Nxyt;
 
-  // stage the tile's update list
-  for (int i = threadIdx.x; i <= kTileH; i += blockDim.x) s_hoff[i] = off[i] - base;
-  for (int i = threadIdx.x; i < kTileCells; i += blockDim.x) s_pos[i] = -1;
-  __syncthreads();
-  const int n = s_hoff[kTileH];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int32_t c = a.cells[base + i];
-    const uint16_t loc = a.local[base + i];
-    s_cell[i] = c;
-    s_code[i] = a.code[c];
-    s_local[i] = loc;
-    const int la = loc & 31, lb = (loc >> 5) & 31, lc = loc >> 10;
-    s_pos[(lc * kTile + lb) * kTile + la] = (int16_t)(i - s_hoff[la + lb + lc]);
+  int64_t cell[kSweepBatch];
+  int oct[kSweepBatch], chk[kSweepBatch];
+  uint32_t flg[kSweepBatch];
+  bool act[kSweepBatch];
+  double r0[kSweepBatch], u[kSweepBatch], v[kSweepBatch], ww[kSweepBatch];
+  T fp[kSweepBatch], nb[kSweepBatch][6];
+#pragma unroll
+  for (int k = 0; k < kSweepBatch; ++k) {
+    const int64_t w = wv * kSweepBatch + k;
+    act[k] = w < nwork;
+    const int64_t t = (act[k] ? w : wv * kSweepBatch) / nch;
+    chk[k] = (int)((act[k] ? w : wv * kSweepBatch) - t * nch);
+    const uint64_t task = a.tasks[t];
+    cell[k] = (int64_t)(uint32_t)task;
+    oct[k] = (int)(task >> 32) & 7;
+    flg[k] = (uint32_t)(task >> 35) & 15u;
   }
-  __syncthreads();
-
-  const int per_round = nwaves * kRound;
-  auto rounds = [&](int h) { return (s_hoff[h + 1] - s_hoff[h] + per_round - 1) / per_round; };
-
-  // loads for round r of hyperplane h into p
-  auto prefetch = [&](Pending<T>* p, int h, int r) {
-    const int hs = s_hoff[h], he = s_hoff[h + 1];
 #pragma unroll
-    for (int k = 0; k < kRound; ++k) {
-      const int e = hs + (r * kRound + k) * nwaves + wave;
-      const int es = e < he ? e : hs;
-      p[k].e = e < he ? e : -1;
-      const int32_t cell = s_cell[es];
-      const uint32_t cc = s_code[es];
-      const uint16_t loc = s_local[es];
-      p[k].cell = cell;
-      p[k].code = cc;
-      p[k].r = a.rho[cell]; p[k].u = a.ux[cell]; p[k].v = a.uy[cell]; p[k].w = a.uz[cell];
-      p[k].fp = fo[(int64_t)cell * Nc8];
-      const int lc3[3] = {loc & 31, (loc >> 5) & 31, loc >> 10};
-      const bool view = (cc >> (24 + o)) & 1u;
+  for (int k = 0; k < kSweepBatch; ++k) {
+    const int o = oct[k];
+    const int l = chk[k] * kWave + lane;
+    const int ll = l < Nc8 ? l : 0;
+    const int64_t dx = osx(o), dy = osy(o) * Nxt, dz = osz(o) * Nxyt;
+    const T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8 + ll;
+    const int64_t c = cell[k];
+    r0[k] = a.rho[c]; u[k] = a.ux[c]; v[k] = a.uy[c]; ww[k] = a.uz[c];
+    fp[k] = fo[c * Nc8];
+    nb[k][0] = fo[(c - dx) * Nc8];
+    nb[k][1] = fo[(c - 2 * dx) * Nc8];
+    nb[k][2] = fo[(c - dy) * Nc8];
+    nb[k][3] = fo[(c - 2 * dy) * Nc8];
+    nb[k][4] = fo[(c - dz) * Nc8];
+    nb[k][5] = fo[(c - 2 * dz) * Nc8];
+  }
 #pragma unroll
-      for (int d = 0; d < 6; ++d) {
-        const int ax = d >> 1, dist = (d & 1) + 1;
-        int q[3] = {lc3[0], lc3[1], lc3[2]};
-        q[ax] -= dist;
-        int slot = -1;
-        if (q[ax] >= 0) {
-          const int ps = s_pos[(q[2] * kTile + q[1]) * kTile + q[0]];
-          if (ps >= 0) slot = ((h - dist) % 3) * kTileHMax + ps;
-        }
-        p[k].ring[d] = (int16_t)slot;
-        if (slot < 0) {
-          const int64_t nbr = (int64_t)cell - dist * step[ax];
-          const T* src = fo + nbr * Nc8;
-          if (view && ax > 0) {
-            const T* q2 = switched_source(a, nbr, o, ax);
-            if (q2) src = q2 + ll;
-          }
-          p[k].nb[d] = *src;
-        } else {
-          p[k].nb[d] = (T)0;
-        }
+  for (int k = 0; k < kSweepBatch; ++k) {
+    if (flg[k] & 8u) {  // reads a wall slot the reference switched
+      const int o = oct[k];
+      const int l = chk[k] * kWave + lane;
+      const int ll = l < Nc8 ? l : 0;
+      const int64_t dy = osy(o) * Nxt, dz = osz(o) * Nxyt;
+      const T* q;
+      if ((q = switched_source(a, cell[k] - dy, o, 1))) nb[k][2] = q[ll];
+      if ((q = switched_source(a, cell[k] - 2 * dy, o, 1))) nb[k][3] = q[ll];
+      if ((q = switched_source(a, cell[k] - dz, o, 2))) nb[k][4] = q[ll];
+      if ((q = switched_source(a, cell[k] - 2 * dz, o, 2))) nb[k][5] = q[ll];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kSweepBatch; ++k) {
+    const int o = oct[k];
+    const int sx = osx(o), sy = osy(o), sz = osz(o);
+    const int l = chk[k] * kWave + lane;
+    const bool lact = l < Nc8;
+    const int ll = lact ? l : 0;
+    const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
+    const bool fx = flg[k] & 1u, fy = flg[k] & 2u, fz = flg[k] & 4u;
+    double tt = sx > 0 ? cxl * u[k] : -(cxl * u[k]);
+    tt = sy > 0 ? tt + cyl * v[k] : tt - cyl * v[k];
+    tt = sz > 0 ? tt + czl * ww[k] : tt - czl * ww[k];
+    const double feq = wl * (r0[k] + 1.0 * tt);
+    double num = a.mu * (feq - 0.5 * (double)fp[k]);
+    num = num + cxl * (fx ? a.d1 : a.c1) * (double)nb[k][0];
+    num = num + cxl * (fx ? 0.0 : a.c2) * (double)nb[k][1];
+    num = num + cyl * (fy ? a.d1 : a.c1) * (double)nb[k][2];
+    num = num + cyl * (fy ? 0.0 : a.c2) * (double)nb[k][3];
+    num = num + czl * (fz ? a.d1 : a.c1) * (double)nb[k][4];
+    num = num + czl * (fz ? 0.0 : a.c2) * (double)nb[k][5];
+    double den = 0.5 * a.mu + cxl * (fx ? a.d1 : a.c0);
+    den = den + cyl * (fy ? a.d1 : a.c0);
+    den = den + czl * (fz ? a.d1 : a.c0);
+    const T fnew = (T)(num / den);
+    T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8 + ll;
+    if (act[k] && lact) fo[cell[k] * Nc8] = fnew;
+    if (a.part) {
+      const double fv = lact ? (double)fnew : 0.0;
+      const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv, lane);
+      if (act[k] && (lane & 15) == 0) {
+        const int q = lane >> 4;
+        const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
+        a.part[(((size_t)cell[k] * 8 + o) * nch + chk[k]) * 4 + q] = sgn * z;
       }
     }
-  };
-
-  // first round that has cells, from hyperplane h on
-  auto first_from = [&](int h, int* hh) {
-    for (; h < kTileH; ++h)
-      if (s_hoff[h + 1] > s_hoff[h]) { *hh = h; return true; }
-    return false;
-  };
-
-  Pending<T> cur[kRound], nxt[kRound];
-  int h0 = 0;
-  if (first_from(0, &h0)) prefetch(cur, h0, 0);
-  const double den0 = 0.5 * a.mu;
-
-  for (int h = 0; h < kTileH; ++h) {
-    const int nr = rounds(h);
-    T* rw = ring + (h % 3) * kTileHMax * kWave + lane;
-    for (int r = 0; r < nr; ++r) {
-      // issue the next round's loads before working on this one
-      int hn = h, rn = r + 1;
-      bool more = rn < nr;
-      if (!more) { rn = 0; more = first_from(h + 1, &hn); }
-      if (more) prefetch(nxt, hn, rn);
-#pragma unroll
-      for (int k = 0; k < kRound; ++k) {
-        const Pending<T>& p = cur[k];
-        double nbv[6];
-#pragma unroll
-        for (int d = 0; d < 6; ++d)
-          nbv[d] = p.ring[d] >= 0 ? (double)ring[p.ring[d] * kWave + lane] : (double)p.nb[d];
-        const uint32_t cc = p.code;
-        const bool fx = (cc >> (3 * o)) & 1u, fy = (cc >> (3 * o + 1)) & 1u,
-                   fz = (cc >> (3 * o + 2)) & 1u;
-        const double r0 = p.r, u = p.u, v = p.v, ww = p.w;
-        double t = sx > 0 ? cxl * u : -(cxl * u);
-        t = sy > 0 ? t + cyl * v : t - cyl * v;
-        t = sz > 0 ? t + czl * ww : t - czl * ww;
-        const double feq = wl * (r0 + 1.0 * t);
-        double num = a.mu * (feq - 0.5 * (double)p.fp);
-        num = num + cxl * (fx ? a.d1 : a.c1) * nbv[0];
-        num = num + cxl * (fx ? 0.0 : a.c2) * nbv[1];
-        num = num + cyl * (fy ? a.d1 : a.c1) * nbv[2];
-        num = num + cyl * (fy ? 0.0 : a.c2) * nbv[3];
-        num = num + czl * (fz ? a.d1 : a.c1) * nbv[4];
-        num = num + czl * (fz ? 0.0 : a.c2) * nbv[5];
-        double den = den0 + cxl * (fx ? a.d1 : a.c0);
-        den = den + cyl * (fy ? a.d1 : a.c0);
-        den = den + czl * (fz ? a.d1 : a.c0);
-        const T fnew = (T)(num / den);
-        if (p.e >= 0) {
-          if (lact) fo[(int64_t)p.cell * Nc8] = fnew;
-          rw[(p.e - s_hoff[h]) * kWave] = fnew;
-        }
-        if (a.part) {
-          const double fv = lact ? (double)fnew : 0.0;
-          const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv, lane);
-          if (p.e >= 0 && (lane & 15) == 0) {
-            const int q = lane >> 4;
-            const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
-            a.part[(((size_t)p.cell * 8 + o) * nch + ch) * 4 + q] = sgn * z;
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < kRound; ++k) cur[k] = nxt[k];
-    }
-    __syncthreads();
   }
 }
 
@@ -725,7 +655,6 @@ struct Ctx {
   double* plane = nullptr;
   double *dcx = nullptr, *dcy = nullptr, *dcz = nullptr, *dw = nullptr;
   uint8_t* lab = nullptr;
-  uint32_t* code = nullptr;
   int32_t* wid = nullptr;
   uint64_t* views = nullptr;
   int32_t *wall_cell = nullptr, *wall_corner = nullptr;
@@ -734,14 +663,11 @@ struct Ctx {
   int32_t *uncoupled = nullptr, *coupled = nullptr, *coupled_snap = nullptr, *snap_cell = nullptr;
   int32_t *commit_walls = nullptr;
   int32_t *fluid_int = nullptr, *fluid_bnd = nullptr, *fluid_all = nullptr;
-  // sweep schedule: tiles grouped by diagonal I+J+K (oriented)
-  int ndiag = 0;
+  // sweep schedule: tasks of each oriented hyperplane
   int nchunk = 1;              // 64-velocity chunks
-  std::vector<int> diag_off;   // ndiag + 1
-  int4* tiles = nullptr;
-  int32_t* sweep_cells = nullptr;
-  uint16_t* sweep_local = nullptr;
-  int32_t* sweep_off = nullptr;
+  int batch = 1;               // sweep tasks per wave (1, 2 or 4)
+  bool xcd_remap = true;       // contiguous task ranges per XCD
+  uint64_t* hp_task = nullptr;
   // timing: four events per step of the last piks_iterate call
   std::vector<hipEvent_t> ev;
   double last_ms[4] = {0, 0, 0, 0};
@@ -808,28 +734,6 @@ int setup_velocity(Ctx& c) {
   return 0;
 }
 
-// tiles of the oriented grid grouped by diagonal I+J+K (one launch each)
-int setup_tiles(Ctx& c) {
-  const Geometry& g = c.g;
-  c.ndiag = g.NTX + g.NTY + g.NTZ - 2;
-  std::vector<int4> tl;
-  c.diag_off.clear();
-  for (int dg = 0; dg < c.ndiag; ++dg) {
-    c.diag_off.push_back((int)tl.size());
-    for (int K = 0; K < g.NTZ; ++K)
-      for (int J = 0; J < g.NTY; ++J) {
-        const int I = dg - J - K;
-        if (I >= 0 && I < g.NTX) tl.push_back(make_int4(I, J, K, 0));
-      }
-  }
-  c.diag_off.push_back((int)tl.size());
-  if (int rc = upload(c, &c.tiles, tl)) return rc;
-  if (int rc = upload(c, &c.sweep_cells, g.sweep_cells)) return rc;
-  if (int rc = upload(c, &c.sweep_local, g.sweep_local)) return rc;
-  if (int rc = upload(c, &c.sweep_off, g.sweep_off)) return rc;
-  return 0;
-}
-
 int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) {
   if (!p || !raw || !out) return fail(PIKS_E_ARG, "null argument");
   if (p->precision != PIKS_PREC_FP64 && p->precision != PIKS_PREC_FP32_REFINED)
@@ -850,6 +754,12 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   const Geometry& g = c->g;
   c->d = Dims{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, g.Ntotal, (int)c->cx.size()};
   c->nchunk = (c->d.Nc8 + kWave - 1) / kWave;
+  if (const char* b = std::getenv("PIKS_SWEEP_BATCH")) {
+    const int v = std::atoi(b);
+    if (v != 1 && v != 2 && v != 4) return fail(PIKS_E_ARG, "PIKS_SWEEP_BATCH must be 1, 2 or 4");
+    c->batch = v;
+  }
+  if (const char* b = std::getenv("PIKS_XCD_REMAP")) c->xcd_remap = std::atoi(b) != 0;
   c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
@@ -862,7 +772,7 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   size_t free_b = 0, total_b = 0;
   HIP_TRY(hipMemGetInfo(&free_b, &total_b));
   const size_t need = nf * (sizeof(double) + (refined ? sizeof(float) : 0)) +
-                      (size_t)g.Ntotal * (32 * 8 * c->nchunk + 4 * 8 + 16) + g.sweep_cells.size() * 6;
+                      (size_t)g.Ntotal * (32 * 8 * c->nchunk + 4 * 8 + 16) + g.hp_task.size() * 8;
   if (need > free_b)
     return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
                                 std::to_string(free_b >> 20) + " MiB free");
@@ -887,7 +797,7 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   if ((rc = upload(*c, &c->dcx, c->cx)) || (rc = upload(*c, &c->dcy, c->cy)) ||
       (rc = upload(*c, &c->dcz, c->cz)) || (rc = upload(*c, &c->dw, c->w)))
     return rc;
-  if ((rc = upload(*c, &c->lab, g.lab)) || (rc = upload(*c, &c->code, g.code)) ||
+  if ((rc = upload(*c, &c->lab, g.lab)) ||
       (rc = upload(*c, &c->wid, g.wid)) || (rc = upload(*c, &c->views, g.wall_views)) ||
       (rc = upload(*c, &c->wall_cell, g.wall_cell)) || (rc = upload(*c, &c->wall_corner, g.wall_corner)) ||
       (rc = upload(*c, &c->wall_label, g.wall_label)) || (rc = upload(*c, &c->e1, g.wall_e1)) ||
@@ -899,9 +809,7 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   std::vector<int32_t> all(g.fluid_interior);
   all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
   if ((rc = upload(*c, &c->fluid_all, all))) return rc;
-  if (g.TX != kTile || g.TY != kTile || g.TZ != kTile || g.nh != kTileH)
-    return fail(PIKS_E_STATE, "sweep schedule does not match the kernel's tile");
-  if ((rc = setup_tiles(*c))) return rc;
+  if ((rc = upload(*c, &c->hp_task, g.hp_task))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c.release();
   return 0;
@@ -926,10 +834,9 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
   const Geometry& g = c.g;
   const Dims d = c.d;
   SweepArgs<T> sa{};
-  sa.f = st.f; sa.lab = c.lab; sa.code = c.code;
-  sa.cells = c.sweep_cells; sa.local = c.sweep_local; sa.off = c.sweep_off;
-  sa.NTY = g.NTY; sa.NTZ = g.NTZ; sa.nh = g.nh; sa.nchunk = c.nchunk;
-  sa.NT = (int64_t)g.NTX * g.NTY * g.NTZ;
+  sa.f = st.f; sa.lab = c.lab;
+  sa.nchunk = c.nchunk;
+  sa.xcd_remap = c.xcd_remap;
   sa.rho = c.rho; sa.ux = c.ux; sa.uy = c.uy; sa.uz = c.uz;
   sa.part = c.exact ? nullptr : c.part;
   sa.wid = c.wid; sa.views = c.views; sa.wall_corner = c.wall_corner; sa.fw = st.fw;
@@ -947,10 +854,14 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
   for (int step = 0; step < nsteps; ++step) {
     const bool tm = step < timed;
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 0], c.stream));
-    for (int dg = 0; dg < c.ndiag; ++dg) {
-      const int nt = c.diag_off[dg + 1] - c.diag_off[dg];
-      sa.tiles = c.tiles + c.diag_off[dg];
-      hipLaunchKernelGGL(k_sweep<T>, dim3(nt * 8 * c.nchunk), dim3(kSweepThreads), 0, c.stream, sa);
+    for (int h = 0; h < g.nhp; ++h) {
+      sa.ntask = g.hp_off[h + 1] - g.hp_off[h];
+      if (sa.ntask == 0) continue;
+      sa.tasks = c.hp_task + g.hp_off[h];
+      const int64_t waves = (sa.ntask * c.nchunk + c.batch - 1) / c.batch, wpg = kSweepThreads / kWave;
+      const int64_t blocks = ((waves + wpg - 1) / wpg + 7) / 8 * 8;
+      auto kern = c.batch == 1 ? k_sweep<T, 1> : c.batch == 2 ? k_sweep<T, 2> : k_sweep<T, 4>;
+      hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kSweepThreads), 0, c.stream, sa);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
