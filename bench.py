@@ -86,7 +86,7 @@ def traffic_from_profiles(workload_name):
     path = os.path.join(HERE, "profiles", "traffic_%s.json" % workload_name)
     if os.path.exists(path):
         with open(path) as fh:
-            return json.load(fh).get("bytes_per_launch")
+            return json.load(fh).get("bytes_per_step")
     return None
 
 

@@ -328,7 +328,7 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
       }
     g.coupled_level_off.push_back((int32_t)g.coupled.size());
   }
-  if (g.Ntotal >= ((int64_t)1 << 31)) {
+  if (g.Ntotal >= ((int64_t)1 << 28)) {
     err = "domain of " + std::to_string(g.Ntotal) + " cells (with ghosts) is too large";
     return PIKS_E_ARG;
   }
@@ -337,40 +337,87 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
 }
 
 void build_sweep_schedule(Geometry& g) {
+  // cells that are stored
+  std::vector<uint8_t> stored((size_t)g.Ntotal, 0);
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = 0; i <= g.Nx + 1; ++i) {
+        const uint8_t l = g.lab[(size_t)g.cell(i, j, k)];
+        const bool plane = i <= 1 || i >= g.Nx || j == 1 || j == g.Ny || k == 1 || k == g.Nz;
+        if (plane || l == FLUID || l >= 20) stored[(size_t)g.cell(i, j, k)] = 1;
+      }
+  int64_t ns = 0;
+  for (uint8_t v : stored) ns += v;
+  g.nstore = ns;
+  // family orders: counting sort on sx*i + sy*j + sz*k, stable over (k, j)
+  const int kmin = -(g.Nx + 2) - (g.Ny + 2) - (g.Nz + 2), nkey = 2 * (-kmin) + 1;
+  for (int F = 0; F < 4; ++F) {
+    const int sx = osx(F), sy = osy(F), sz = osz(F);
+    std::vector<int64_t> start((size_t)nkey + 1, 0);
+    for (int k = -1; k <= g.Nz + 2; ++k)
+      for (int j = -1; j <= g.Ny + 2; ++j)
+        for (int i = -1; i <= g.Nx + 2; ++i)
+          if (stored[(size_t)g.cell(i, j, k)]) ++start[(size_t)(sx * i + sy * j + sz * k - kmin) + 1];
+    for (int q = 0; q < nkey; ++q) start[(size_t)q + 1] += start[(size_t)q];
+    std::vector<int32_t>& pos = g.pos[F];
+    pos.assign((size_t)g.Ntotal, (int32_t)ns);
+    for (int k = -1; k <= g.Nz + 2; ++k)
+      for (int j = -1; j <= g.Ny + 2; ++j)
+        for (int i = -1; i <= g.Nx + 2; ++i) {
+          const int64_t c = g.cell(i, j, k);
+          if (stored[(size_t)c]) pos[(size_t)c] = (int32_t)start[(size_t)(sx * i + sy * j + sz * k - kmin)]++;
+        }
+  }
+
+  // tasks, bucketed by (hyperplane, octant); raster order (k, j) inside a
+  // bucket is ascending storage order
   g.nhp = (g.Nx - 1) + (g.Ny - 1) + (g.Nz - 1) + 1;
   const int64_t nb = (int64_t)g.nhp * 8;
   std::vector<int64_t> count((size_t)nb + 1, 0);
-  auto oriented = [&](int o, int i, int j, int k, int* oi, int* oj, int* ok) {
-    *oi = osx(o) > 0 ? i - 1 : g.Nx - i;
-    *oj = osy(o) > 0 ? j - 1 : g.Ny - j;
-    *ok = osz(o) > 0 ? k - 1 : g.Nz - k;
-    return *oi >= 1 && *oj >= 1 && *ok >= 1;  // the octant's inflow layer is not swept
+  auto oriented = [&](int o, int i, int j, int k, int* s) {
+    const int oi = osx(o) > 0 ? i - 1 : g.Nx - i;
+    const int oj = osy(o) > 0 ? j - 1 : g.Ny - j;
+    const int ok = osz(o) > 0 ? k - 1 : g.Nz - k;
+    *s = oi + oj + ok;
+    return oi >= 1 && oj >= 1 && ok >= 1;  // the octant's inflow layer is not swept
   };
+  const int W = Geometry::kTaskWords;
   for (int pass = 0; pass < 2; ++pass) {
-    std::vector<int64_t> pos;
+    std::vector<int64_t> at;
     if (pass == 1) {
-      g.hp_off.assign((size_t)g.nhp + 1, 0);
       for (int64_t b = 0; b < nb; ++b) count[(size_t)b + 1] += count[(size_t)b];
+      g.hp_off.assign((size_t)g.nhp + 1, 0);
       for (int s = 0; s <= g.nhp; ++s) g.hp_off[(size_t)s] = count[(size_t)s * 8];
-      g.hp_task.assign((size_t)count[(size_t)nb], 0);
-      pos.assign(count.begin(), count.end() - 1);
+      g.hp_task.assign((size_t)count[(size_t)nb] * W, 0);
+      at.assign(count.begin(), count.end() - 1);
     }
-    for (int o = 0; o < 8; ++o)
+    for (int o = 0; o < 8; ++o) {
+      const std::vector<int32_t>& pos = g.pos[ofam(o)];
+      const int64_t dx = osx(o), dy = (int64_t)osy(o) * g.Nxt, dz = (int64_t)osz(o) * g.Nxyt;
       for (int k = 1; k <= g.Nz; ++k)
         for (int j = 1; j <= g.Ny; ++j)
           for (int i = 1; i <= g.Nx; ++i) {
             const int64_t c = g.cell(i, j, k);
-            int oi, oj, ok;
-            if (g.lab[(size_t)c] != FLUID || !oriented(o, i, j, k, &oi, &oj, &ok)) continue;
-            const int64_t b = (int64_t)(oi + oj + ok) * 8 + o;
-            if (pass == 0)
+            int s;
+            if (g.lab[(size_t)c] != FLUID || !oriented(o, i, j, k, &s)) continue;
+            const int64_t b = (int64_t)s * 8 + o;
+            if (pass == 0) {
               ++count[(size_t)b + 1];
-            else {
-              const uint32_t cc = g.code[(size_t)c];
-              const uint64_t flags = ((cc >> (3 * o)) & 7u) | (((cc >> (24 + o)) & 1u) << 3);
-              g.hp_task[(size_t)pos[(size_t)b]++] = (uint64_t)c | (uint64_t)o << 32 | flags << 35;
+              continue;
             }
+            int32_t* t = &g.hp_task[(size_t)(at[(size_t)b]++ * W)];
+            const uint32_t cc = g.code[(size_t)c];
+            const uint32_t flags = ((cc >> (3 * o)) & 7u) | (((cc >> (24 + o)) & 1u) << 3);
+            t[0] = (int32_t)((uint32_t)pos[(size_t)c] | (uint32_t)o << 28);
+            t[1] = pos[(size_t)(c - dx)];
+            t[2] = pos[(size_t)(c - 2 * dx)];
+            t[3] = pos[(size_t)(c - dy)];
+            t[4] = pos[(size_t)(c - 2 * dy)];
+            t[5] = pos[(size_t)(c - dz)];
+            t[6] = pos[(size_t)(c - 2 * dz)];
+            t[7] = (int32_t)((uint32_t)c | flags << 28);
           }
+    }
   }
 }
 

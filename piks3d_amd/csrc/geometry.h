@@ -54,17 +54,31 @@ struct Geometry {
   std::vector<int32_t> commit_walls;      // walls with a switch in some octant
   int64_t nfluid = 0;
 
+  // Storage of f.  Per octant only the cells that some kernel writes are
+  // kept: fluid and wall cells, the x ghost planes 0 and Nx+1, and the
+  // boundary planes x = 1, Nx, y = 1, Ny, z = 1, Nz.  Every other cell
+  // (interior solid, y/z ghosts, x ghosts -1 and Nx+2) is zero throughout and
+  // reads from one extra zero slot at index nstore.  Octant o keeps its cells
+  // in the order of its family ofam(o): ascending sx*i + sy*j + sz*k with the
+  // family's signs, then k, then j, so that one hyperplane of a sweep is one
+  // contiguous range.  pos[F] maps a cell to its index in family F's order.
+  int64_t nstore = 0;
+  std::vector<int32_t> pos[4];
+
   // Sweep schedule.  Octant o updates its swept cells (fluid cells past the
   // octant's inflow layer) in order of the oriented hyperplane
   // s = oi + oj + ok, where oi = i-1 when the octant's cx > 0 and Nx-i when
   // cx < 0 (likewise oj, ok): a cell's upwind neighbours lie on s-1 and s-2,
   // so one hyperplane of all eight octants can be updated at once.
-  // hp_task[hp_off[s] .. hp_off[s+1]) lists the tasks of hyperplane s,
-  // octant-major, raster order within an octant.  A task word holds the cell
-  // (bits 0-31), the octant (32-34), the octant's first-order flags for x, y,
-  // z (35-37) and its "reads a switched wall" flag (38).
+  // hp_task[8t .. 8t+8) describes task t: [0] storage index | octant << 28,
+  // [1..6] storage indices of the upwind neighbours at distance 1 and 2 along
+  // x, y, z, [7] cell | flags << 28, flags = the octant's first-order flags
+  // for x, y, z (bits 0-2) and its "reads a switched wall" flag (bit 3).
+  // Tasks of hyperplane s are hp_off[s] .. hp_off[s+1), octant-major, each
+  // octant's tasks in ascending storage order.
+  static constexpr int kTaskWords = 8;
   int nhp = 0;
-  std::vector<uint64_t> hp_task;
+  std::vector<int32_t> hp_task;
   std::vector<int64_t> hp_off;
 
   int64_t cell(int i, int j, int k) const {
@@ -76,7 +90,7 @@ struct Geometry {
 int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geometry& g,
                    std::string& err);
 
-// Fill the hyperplane sweep schedule (hp_task, hp_off).
+// Fill the storage order (nstore, pos) and the sweep schedule (hp_task, hp_off).
 void build_sweep_schedule(Geometry& g);
 
 }  // namespace piks

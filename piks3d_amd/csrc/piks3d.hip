@@ -70,18 +70,47 @@ struct Dims {
   int Nx, Ny, Nz, Nxt;
   int64_t Nxyt, Ntotal;
   int Nc8;
+  // storage of f: octant o keeps its cells in family ofam(o)'s order, pos[F]
+  // maps a cell to that order (unstored cells to the zero slot); ostride =
+  // elements per octant
+  const int32_t* pos[4];
+  int64_t ostride;
 };
+
+// Moments of one cell (src/flow.F90:72 keeps Rho, Ux, Uy, Uz as four
+// arrays); interleaved so that a task loads them with one scalar load.
+struct alignas(32) Mom {
+  double r, x, y, z;
+};
+
+// Sweep task record (see Geometry::hp_task), loaded with one scalar load.
+struct alignas(32) TaskRec {
+  int32_t w[8];
+};
+
+// Read-only data of a kernel loaded through the scalar cache: the constant
+// address space tells the compiler that no store of the kernel can clobber it.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(4))) T*)p;
+#else
+  return *p;  // host pass: never called
+#endif
+}
+
+// element offset of (octant o, cell, velocity 0) in f
+__device__ __forceinline__ size_t fat(const Dims& d, int o, int64_t cell) {
+  return (size_t)o * d.ostride + (size_t)d.pos[ofam(o)][cell] * d.Nc8;
+}
 
 // ------------------------------------------------------------------ sweep
 
 template <typename T>
 struct SweepArgs {
   T* f;
-  const double* rho;
-  const double* ux;
-  const double* uy;
-  const double* uz;
-  double* part;            // [cell][octant][chunk][4] partial moments; null = skip
+  const Mom* mom;
+  double* part;            // [cell][octant][4] partial moments; null = skip
   const int32_t* wid;
   const uint64_t* views;
   const int32_t* wall_corner;
@@ -93,13 +122,11 @@ struct SweepArgs {
   const double* w;
   double mu, c0, c1, c2, d1;  // 1.5/ds, 2/ds, -0.5/ds, 1/ds
   Dims d;
-  const uint64_t* tasks;   // this hyperplane's task words (cell, octant, flags)
+  const TaskRec* tasks;    // this hyperplane's task records
   int64_t ntask;
-  int nchunk;
   bool xcd_remap;
 };
 
-constexpr int kSweepThreads = 256;
 
 // Value a cell reading neighbour n along `axis` (1 = y, 2 = z) sees in octant
 // o, when the reference would have switched that wall slot by then; null if
@@ -111,7 +138,7 @@ __device__ __forceinline__ const T* switched_source(const SweepArgs<T>& a, int64
   const int w = a.wid[n];
   const int src = (int)((a.views[w] >> (8 * o + (axis == 2 ? 4 : 0))) & 15u);
   if (src == 0) return nullptr;
-  if (src <= 8) return a.f + ((size_t)(src - 1) * a.d.Ntotal + n) * a.d.Nc8;
+  if (src <= 8) return a.f + fat(a.d, src - 1, n);
   return a.fw + ((size_t)a.wall_corner[w] * 8 + o) * a.d.Nc8;
 }
 
@@ -143,107 +170,74 @@ __device__ __forceinline__ int64_t xcd_block(bool remap) {
   return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
-// One hyperplane of the eight octant sweeps.  A wave updates kSweepBatch
-// (B) (cell, octant, chunk of 64 velocities) tasks: their upwind neighbours lie
-// on the two previous hyperplanes (earlier launches), so no wave waits on
-// another; the loads of all its tasks are issued before any is used.
+// One hyperplane of the eight octant sweeps.  Every task (cell, octant) has
+// its upwind neighbours on the two previous hyperplanes (earlier launches), so
+// no wave waits on another.  A wave updates one task, all its velocities (64
+// per pass); the task record holds the storage indices of the cell and of its
+// six upwind neighbours, so all f loads are issued at once.
 // Each update is src/solver.F90:206-214 (and analogues) with the same
 // operations in the same order (contraction off): signs enter as exact
 // negations, and a first-order axis contributes (c*0)*f like the reference.
-template <typename T, int kSweepBatch>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs<T> a) {
-  const int nch = a.nchunk;
+template <typename T, int kThreads>
+__global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
   // the wave index is uniform over the wave: keep it (and everything derived
-  // from it: tasks, cells, octants, addresses' uniform parts) in SGPRs
+  // from it: the record, cell, octant, the uniform parts of addresses) in SGPRs
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int64_t wv = xcd_block(a.xcd_remap) * (kSweepThreads / kWave) + wave;
-  const int64_t nwork = a.ntask * nch;
-  if (wv * kSweepBatch >= nwork) return;
+  const int64_t t = xcd_block(a.xcd_remap) * (kThreads / kWave) + wave;
+  if (t >= a.ntask) return;
   const int lane = threadIdx.x & (kWave - 1);
+  const TaskRec r = ld_const(a.tasks + t);
+  const int o = (int)((uint32_t)r.w[0] >> 28);
+  const int64_t self = r.w[0] & 0x0fffffff;
+  const int64_t cell = r.w[7] & 0x0fffffff;
+  const uint32_t flg = (uint32_t)r.w[7] >> 28;
+  const Mom m = ld_const(a.mom + cell);
+  const int sx = osx(o), sy = osy(o), sz = osz(o);
   const int Nc8 = a.d.Nc8;
-  const int64_t Nxt = a.d.Nxt, Nxyt = a.d.Nxyt;
-
-  int64_t cell[kSweepBatch];
-  int oct[kSweepBatch], chk[kSweepBatch];
-  uint32_t flg[kSweepBatch];
-  bool act[kSweepBatch];
-  double r0[kSweepBatch], u[kSweepBatch], v[kSweepBatch], ww[kSweepBatch];
-  T fp[kSweepBatch], nb[kSweepBatch][6];
-#pragma unroll
-  for (int k = 0; k < kSweepBatch; ++k) {
-    const int64_t w = wv * kSweepBatch + k;
-    act[k] = w < nwork;
-    const int64_t t = (act[k] ? w : wv * kSweepBatch) / nch;
-    chk[k] = (int)((act[k] ? w : wv * kSweepBatch) - t * nch);
-    const uint64_t task = a.tasks[t];
-    cell[k] = (int64_t)(uint32_t)task;
-    oct[k] = (int)(task >> 32) & 7;
-    flg[k] = (uint32_t)(task >> 35) & 15u;
-  }
-#pragma unroll
-  for (int k = 0; k < kSweepBatch; ++k) {
-    const int o = oct[k];
-    const int l = chk[k] * kWave + lane;
-    const int ll = l < Nc8 ? l : 0;
-    const int64_t dx = osx(o), dy = osy(o) * Nxt, dz = osz(o) * Nxyt;
-    const T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8 + ll;
-    const int64_t c = cell[k];
-    r0[k] = a.rho[c]; u[k] = a.ux[c]; v[k] = a.uy[c]; ww[k] = a.uz[c];
-    fp[k] = fo[c * Nc8];
-    nb[k][0] = fo[(c - dx) * Nc8];
-    nb[k][1] = fo[(c - 2 * dx) * Nc8];
-    nb[k][2] = fo[(c - dy) * Nc8];
-    nb[k][3] = fo[(c - 2 * dy) * Nc8];
-    nb[k][4] = fo[(c - dz) * Nc8];
-    nb[k][5] = fo[(c - 2 * dz) * Nc8];
-  }
-#pragma unroll
-  for (int k = 0; k < kSweepBatch; ++k) {
-    if (flg[k] & 8u) {  // reads a wall slot the reference switched
-      const int o = oct[k];
-      const int l = chk[k] * kWave + lane;
-      const int ll = l < Nc8 ? l : 0;
-      const int64_t dy = osy(o) * Nxt, dz = osz(o) * Nxyt;
-      const T* q;
-      if ((q = switched_source(a, cell[k] - dy, o, 1))) nb[k][2] = q[ll];
-      if ((q = switched_source(a, cell[k] - 2 * dy, o, 1))) nb[k][3] = q[ll];
-      if ((q = switched_source(a, cell[k] - dz, o, 2))) nb[k][4] = q[ll];
-      if ((q = switched_source(a, cell[k] - 2 * dz, o, 2))) nb[k][5] = q[ll];
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kSweepBatch; ++k) {
-    const int o = oct[k];
-    const int sx = osx(o), sy = osy(o), sz = osz(o);
-    const int l = chk[k] * kWave + lane;
+  T* fo = a.f + (size_t)o * a.d.ostride;
+  for (int base = 0; base < Nc8; base += kWave) {
+    const int l = base + lane;
     const bool lact = l < Nc8;
     const int ll = lact ? l : 0;
+    const T fp = fo[self * Nc8 + ll];
+    T nb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nb[q] = fo[(int64_t)r.w[1 + q] * Nc8 + ll];
+    if (flg & 8u) {  // reads a wall slot the reference switched
+      const int64_t dy = (int64_t)sy * a.d.Nxt, dz = (int64_t)sz * a.d.Nxyt;
+      const T* q;
+      if ((q = switched_source(a, cell - dy, o, 1))) nb[2] = q[ll];
+      if ((q = switched_source(a, cell - 2 * dy, o, 1))) nb[3] = q[ll];
+      if ((q = switched_source(a, cell - dz, o, 2))) nb[4] = q[ll];
+      if ((q = switched_source(a, cell - 2 * dz, o, 2))) nb[5] = q[ll];
+    }
     const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
-    const bool fx = flg[k] & 1u, fy = flg[k] & 2u, fz = flg[k] & 4u;
-    double tt = sx > 0 ? cxl * u[k] : -(cxl * u[k]);
-    tt = sy > 0 ? tt + cyl * v[k] : tt - cyl * v[k];
-    tt = sz > 0 ? tt + czl * ww[k] : tt - czl * ww[k];
-    const double feq = wl * (r0[k] + 1.0 * tt);
-    double num = a.mu * (feq - 0.5 * (double)fp[k]);
-    num = num + cxl * (fx ? a.d1 : a.c1) * (double)nb[k][0];
-    num = num + cxl * (fx ? 0.0 : a.c2) * (double)nb[k][1];
-    num = num + cyl * (fy ? a.d1 : a.c1) * (double)nb[k][2];
-    num = num + cyl * (fy ? 0.0 : a.c2) * (double)nb[k][3];
-    num = num + czl * (fz ? a.d1 : a.c1) * (double)nb[k][4];
-    num = num + czl * (fz ? 0.0 : a.c2) * (double)nb[k][5];
+    const bool fx = flg & 1u, fy = flg & 2u, fz = flg & 4u;
+    double tt = sx > 0 ? cxl * m.x : -(cxl * m.x);
+    tt = sy > 0 ? tt + cyl * m.y : tt - cyl * m.y;
+    tt = sz > 0 ? tt + czl * m.z : tt - czl * m.z;
+    const double feq = wl * (m.r + 1.0 * tt);
+    double num = a.mu * (feq - 0.5 * (double)fp);
+    num = num + cxl * (fx ? a.d1 : a.c1) * (double)nb[0];
+    num = num + cxl * (fx ? 0.0 : a.c2) * (double)nb[1];
+    num = num + cyl * (fy ? a.d1 : a.c1) * (double)nb[2];
+    num = num + cyl * (fy ? 0.0 : a.c2) * (double)nb[3];
+    num = num + czl * (fz ? a.d1 : a.c1) * (double)nb[4];
+    num = num + czl * (fz ? 0.0 : a.c2) * (double)nb[5];
     double den = 0.5 * a.mu + cxl * (fx ? a.d1 : a.c0);
     den = den + cyl * (fy ? a.d1 : a.c0);
     den = den + czl * (fz ? a.d1 : a.c0);
     const T fnew = (T)(num / den);
-    T* fo = a.f + (size_t)o * a.d.Ntotal * Nc8 + ll;
-    if (act[k] && lact) fo[cell[k] * Nc8] = fnew;
+    if (lact) fo[self * Nc8 + ll] = fnew;
     if (a.part) {
       const double fv = lact ? (double)fnew : 0.0;
       const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv, lane);
-      if (act[k] && (lane & 15) == 0) {
+      if ((lane & 15) == 0) {
         const int q = lane >> 4;
         const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
-        a.part[(((size_t)cell[k] * 8 + o) * nch + chk[k]) * 4 + q] = sgn * z;
+        double* pp = a.part + ((size_t)cell * 8 + o) * 4 + q;
+        // velocity passes after the first add to the first pass's sum
+        *pp = base == 0 ? sgn * z : *pp + sgn * z;
       }
     }
   }
@@ -266,9 +260,9 @@ __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
   for (int o = 0; o < 8; ++o) {
     const int src = (int)((vv >> (8 * o + 4)) & 15u);
     if (src == 0) continue;
-    const T* s = src <= 8 ? f + ((size_t)(src - 1) * d.Ntotal + cell) * d.Nc8
+    const T* s = src <= 8 ? f + fat(d, src - 1, cell)
                           : fw + ((size_t)wall_corner[w] * 8 + o) * d.Nc8;
-    T* dst = f + ((size_t)o * d.Ntotal + cell) * d.Nc8;
+    T* dst = f + fat(d, o, cell);
     for (int l = lane; l < d.Nc8; l += kWave) dst[l] = s[l];
   }
 }
@@ -291,12 +285,11 @@ __global__ void k_halo(T* f, T* buf_e, T* buf_w, Dims d) {
     const int j = 1 + (int)(r % d.Ny);
     const int k = 1 + (int)(r / d.Ny);
     const int64_t row = (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
-    T* fo = f + (size_t)o * d.Ntotal * d.Nc8;
     const int64_t east = row + d.Nx + 1, west = row + 2;   // x = Nx, x = 1
     const int64_t g0 = row + 1, gN = row + d.Nx + 2;       // x = 0, x = Nx + 1
-    const T ve = fo[east * d.Nc8 + l], vw = fo[west * d.Nc8 + l];
-    fo[g0 * d.Nc8 + l] = buf_e[t];
-    fo[gN * d.Nc8 + l] = buf_w[t];
+    const T ve = f[fat(d, o, east) + l], vw = f[fat(d, o, west) + l];
+    f[fat(d, o, g0) + l] = buf_e[t];
+    f[fat(d, o, gN) + l] = buf_w[t];
     buf_e[t] = ve;
     buf_w[t] = vw;
   }
@@ -331,7 +324,7 @@ template <typename T, bool COUPLED>
 __device__ __forceinline__ double load_nbr(const WallArgs<T>& a, int64_t cell, int o, int l,
                                            int slot) {
   if (COUPLED && slot >= 0) return (double)a.snap[((size_t)slot * 8 + o) * a.d.Nc8 + l];
-  return (double)a.f[((size_t)o * a.d.Ntotal + cell) * a.d.Nc8 + l];
+  return (double)a.f[fat(a.d, o, cell) + l];
 }
 
 // src/solver.F90:772-1823: one wall, all velocities, on one wave.
@@ -412,7 +405,7 @@ __device__ void wall_one(const WallArgs<T>& a, int idx, int lane) {
     for (int s = 0; s < 16; ++s) {
       if (!((written >> s) & 1u)) continue;
       if (s < 8)
-        a.f[((size_t)s * a.d.Ntotal + cell) * Nc8 + l] = (T)v[s];
+        a.f[fat(a.d, s, cell) + l] = (T)v[s];
       else
         a.fw[((size_t)corner * 8 + (s - 8)) * Nc8 + l] = (T)v[s];
     }
@@ -460,7 +453,7 @@ __global__ void k_snapshot(const T* f, T* snap, const int32_t* cells, int n, Dim
     const int l = (int)(t % d.Nc8);
     const int o = (int)((t / d.Nc8) % 8);
     const int s = (int)(t / (8 * d.Nc8));
-    snap[t] = f[((size_t)o * d.Ntotal + cells[s]) * d.Nc8 + l];
+    snap[t] = f[fat(d, o, cells[s]) + l];
   }
 }
 
@@ -482,30 +475,32 @@ __global__ void k_inout(T* f, const double* w, double pd, Dims d) {
     const int64_t row = (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
     const int oin = q == 0 ? 0 : q == 1 ? 3 : q == 2 ? 4 : 7;
     const int oout = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 5 : 6;
-    T* fi = f + (size_t)oin * d.Ntotal * d.Nc8;
-    T* fo = f + (size_t)oout * d.Ntotal * d.Nc8;
     const double wpd = w[l] * pd;
-    fi[(row + 2) * d.Nc8 + l] = (T)((double)fi[(row + 1) * d.Nc8 + l] + wpd);
-    fo[(row + d.Nx + 1) * d.Nc8 + l] = (T)((double)fo[(row + d.Nx + 2) * d.Nc8 + l] - wpd);
+    f[fat(d, oin, row + 2) + l] = (T)((double)f[fat(d, oin, row + 1) + l] + wpd);
+    f[fat(d, oout, row + d.Nx + 1) + l] = (T)((double)f[fat(d, oout, row + d.Nx + 2) + l] - wpd);
   }
 }
 
 // src/solver.F90:1867-1892: y = 1 (south) then y = Ny (north) mirror copies.
 template <typename T>
+__device__ __forceinline__ void copy_slot(T* f, const Dims& d, int64_t cell, int l, int dst, int src) {
+  f[fat(d, dst, cell) + l] = f[fat(d, src, cell) + l];
+}
+
+template <typename T>
 __global__ void k_sym_y(T* f, Dims d) {
   const int64_t n = (int64_t)d.Nx * d.Nz * d.Nc8;
-  const size_t os = (size_t)d.Ntotal * d.Nc8;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
     const int i = 1 + (int)((t / d.Nc8) % d.Nx);
     const int k = 1 + (int)(t / ((int64_t)d.Nc8 * d.Nx));
-    const size_t s = ((size_t)(i + 1) + 2 * (size_t)d.Nxt + (size_t)(k + 1) * d.Nxyt) * d.Nc8 + l;
-    f[1 * os + s] = f[2 * os + s]; f[0 * os + s] = f[3 * os + s];
-    f[5 * os + s] = f[6 * os + s]; f[4 * os + s] = f[7 * os + s];
-    const size_t q = ((size_t)(i + 1) + (size_t)(d.Ny + 1) * d.Nxt + (size_t)(k + 1) * d.Nxyt) * d.Nc8 + l;
-    f[2 * os + q] = f[1 * os + q]; f[3 * os + q] = f[0 * os + q];
-    f[6 * os + q] = f[5 * os + q]; f[7 * os + q] = f[4 * os + q];
+    const int64_t s = (int64_t)(i + 1) + 2 * (int64_t)d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+    copy_slot(f, d, s, l, 1, 2); copy_slot(f, d, s, l, 0, 3);
+    copy_slot(f, d, s, l, 5, 6); copy_slot(f, d, s, l, 4, 7);
+    const int64_t q = (int64_t)(i + 1) + (int64_t)(d.Ny + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+    copy_slot(f, d, q, l, 2, 1); copy_slot(f, d, q, l, 3, 0);
+    copy_slot(f, d, q, l, 6, 5); copy_slot(f, d, q, l, 7, 4);
   }
 }
 
@@ -513,18 +508,17 @@ __global__ void k_sym_y(T* f, Dims d) {
 template <typename T>
 __global__ void k_sym_z(T* f, Dims d) {
   const int64_t n = (int64_t)d.Nx * d.Ny * d.Nc8;
-  const size_t os = (size_t)d.Ntotal * d.Nc8;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
     const int i = 1 + (int)((t / d.Nc8) % d.Nx);
     const int j = 1 + (int)(t / ((int64_t)d.Nc8 * d.Nx));
-    const size_t s = ((size_t)(i + 1) + (size_t)(j + 1) * d.Nxt + 2 * (size_t)d.Nxyt) * d.Nc8 + l;
-    f[0 * os + s] = f[4 * os + s]; f[1 * os + s] = f[5 * os + s];
-    f[2 * os + s] = f[6 * os + s]; f[3 * os + s] = f[7 * os + s];
-    const size_t q = ((size_t)(i + 1) + (size_t)(j + 1) * d.Nxt + (size_t)(d.Nz + 1) * d.Nxyt) * d.Nc8 + l;
-    f[4 * os + q] = f[0 * os + q]; f[5 * os + q] = f[1 * os + q];
-    f[6 * os + q] = f[2 * os + q]; f[7 * os + q] = f[3 * os + q];
+    const int64_t s = (int64_t)(i + 1) + (int64_t)(j + 1) * d.Nxt + 2 * (int64_t)d.Nxyt;
+    copy_slot(f, d, s, l, 0, 4); copy_slot(f, d, s, l, 1, 5);
+    copy_slot(f, d, s, l, 2, 6); copy_slot(f, d, s, l, 3, 7);
+    const int64_t q = (int64_t)(i + 1) + (int64_t)(j + 1) * d.Nxt + (int64_t)(d.Nz + 1) * d.Nxyt;
+    copy_slot(f, d, q, l, 4, 0); copy_slot(f, d, q, l, 5, 1);
+    copy_slot(f, d, q, l, 6, 2); copy_slot(f, d, q, l, 7, 3);
   }
 }
 
@@ -532,20 +526,19 @@ __global__ void k_sym_z(T* f, Dims d) {
 
 // Interior fluid cells: their f is final after the sweep, so the moments are
 // the sum of the eight octant partials.
-__global__ void k_moments_partials(const double* part, int nchunk, const int32_t* cells, int64_t n,
-                                   double* rho, double* ux, double* uy, double* uz) {
+__global__ void k_moments_partials(const double* part, const int32_t* cells, int64_t n, Mom* mom) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = cells[t];
-    const double* p = part + (size_t)c * 32 * nchunk;
+    const double* p = part + (size_t)c * 32;
     double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
-    for (int q = 0; q < 8 * nchunk; ++q) {
+    for (int q = 0; q < 8; ++q) {
       r += p[q * 4 + 0];
       x += p[q * 4 + 1];
       y += p[q * 4 + 2];
       z += p[q * 4 + 3];
     }
-    rho[c] = r; ux[c] = x; uy[c] = y; uz[c] = z;
+    mom[c] = Mom{r, x, y, z};
   }
 }
 
@@ -553,56 +546,54 @@ __global__ void k_moments_partials(const double* part, int nchunk, const int32_t
 template <typename T>
 __global__ __launch_bounds__(256) void k_moments_direct(const T* f, const int32_t* cells, int64_t n,
                                                         const double* cx, const double* cy,
-                                                        const double* cz, Dims d, double* rho,
-                                                        double* ux, double* uy, double* uz) {
+                                                        const double* cz, Dims d, Mom* mom) {
   const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (idx >= n) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t c = cells[idx];
-  const size_t os = (size_t)d.Ntotal * d.Nc8;
+  size_t b8[8];
+  for (int o = 0; o < 8; ++o) b8[o] = fat(d, o, c);
   double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
   for (int l = lane; l < d.Nc8; l += kWave) {
-    const size_t b = (size_t)c * d.Nc8 + l;
-    const double f1 = f[b], f2 = f[os + b], f3 = f[2 * os + b], f4 = f[3 * os + b];
-    const double f5 = f[4 * os + b], f6 = f[5 * os + b], f7 = f[6 * os + b], f8 = f[7 * os + b];
+    const double f1 = f[b8[0] + l], f2 = f[b8[1] + l], f3 = f[b8[2] + l], f4 = f[b8[3] + l];
+    const double f5 = f[b8[4] + l], f6 = f[b8[5] + l], f7 = f[b8[6] + l], f8 = f[b8[7] + l];
     r += f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
     x += cx[l] * (f1 - f2 - f3 + f4 + f5 - f6 - f7 + f8);
     y += cy[l] * (f1 + f2 - f3 - f4 + f5 + f6 - f7 - f8);
     z += cz[l] * (f1 + f2 + f3 + f4 - f5 - f6 - f7 - f8);
   }
   r = wave_sum(r); x = wave_sum(x); y = wave_sum(y); z = wave_sum(z);
-  if (lane == 0) { rho[c] = r; ux[c] = x; uy[c] = y; uz[c] = z; }
+  if (lane == 0) mom[c] = Mom{r, x, y, z};
 }
 
 // The same in the reference's summation order (one thread per cell), for
 // bit-for-bit comparison with the oracle.
 template <typename T>
 __global__ void k_moments_serial(const T* f, const int32_t* cells, int64_t n, const double* cx,
-                                 const double* cy, const double* cz, Dims d, double* rho,
-                                 double* ux, double* uy, double* uz) {
+                                 const double* cy, const double* cz, Dims d, Mom* mom) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t c = cells[t];
-  const size_t os = (size_t)d.Ntotal * d.Nc8;
+  size_t b8[8];
+  for (int o = 0; o < 8; ++o) b8[o] = fat(d, o, c);
   double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
   for (int l = 0; l < d.Nc8; ++l) {
-    const size_t b = (size_t)c * d.Nc8 + l;
-    const double f1 = f[b], f2 = f[os + b], f3 = f[2 * os + b], f4 = f[3 * os + b];
-    const double f5 = f[4 * os + b], f6 = f[5 * os + b], f7 = f[6 * os + b], f8 = f[7 * os + b];
+    const double f1 = f[b8[0] + l], f2 = f[b8[1] + l], f3 = f[b8[2] + l], f4 = f[b8[3] + l];
+    const double f5 = f[b8[4] + l], f6 = f[b8[5] + l], f7 = f[b8[6] + l], f8 = f[b8[7] + l];
     r = r + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
     x = x + cx[l] * (f1 - f2 - f3 + f4 + f5 - f6 - f7 + f8);
     y = y + cy[l] * (f1 + f2 - f3 - f4 + f5 + f6 - f7 - f8);
     z = z + cz[l] * (f1 + f2 + f3 + f4 - f5 - f6 - f7 - f8);
   }
-  rho[c] = r; ux[c] = x; uy[c] = y; uz[c] = z;
+  mom[c] = Mom{r, x, y, z};
 }
 
-__global__ void k_gather_plane(const double* ux, double* out, Dims d, int column) {
+__global__ void k_gather_plane(const Mom* mom, double* out, Dims d, int column) {
   const int64_t n = (int64_t)d.Ny * d.Nz;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int j = 1 + (int)(t % d.Ny), k = 1 + (int)(t / d.Ny);
-    out[t] = ux[(int64_t)(column + 1) + (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt];
+    out[t] = mom[(int64_t)(column + 1) + (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt].x;
   }
 }
 
@@ -650,7 +641,7 @@ struct Ctx {
   State<float> s32;
   int phase = PIKS_PREC_FP64;   // precision the next iterations run in
   size_t n_f = 0, n_fw = 0, n_buf = 0, n_snap = 0;
-  double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
+  Mom* mom = nullptr;
   double* part = nullptr;
   double* plane = nullptr;
   double *dcx = nullptr, *dcy = nullptr, *dcz = nullptr, *dw = nullptr;
@@ -664,10 +655,11 @@ struct Ctx {
   int32_t *commit_walls = nullptr;
   int32_t *fluid_int = nullptr, *fluid_bnd = nullptr, *fluid_all = nullptr;
   // sweep schedule: tasks of each oriented hyperplane
-  int nchunk = 1;              // 64-velocity chunks
-  int batch = 1;               // sweep tasks per wave (1, 2 or 4)
   bool xcd_remap = true;       // contiguous task ranges per XCD
-  uint64_t* hp_task = nullptr;
+  int sweep_block = 64;        // threads per sweep workgroup (one wave: a task's
+                               // workgroup ends with its wave, measured fastest)
+  int32_t* hp_task = nullptr;
+  int32_t* pos[4] = {nullptr, nullptr, nullptr, nullptr};
   // timing: four events per step of the last piks_iterate call
   std::vector<hipEvent_t> ev;
   double last_ms[4] = {0, 0, 0, 0};
@@ -753,17 +745,18 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   if (int rc = build_geometry(gi, raw, nbytes, c->g, err)) return fail(rc, err);
   const Geometry& g = c->g;
   c->d = Dims{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, g.Ntotal, (int)c->cx.size()};
-  c->nchunk = (c->d.Nc8 + kWave - 1) / kWave;
-  if (const char* b = std::getenv("PIKS_SWEEP_BATCH")) {
-    const int v = std::atoi(b);
-    if (v != 1 && v != 2 && v != 4) return fail(PIKS_E_ARG, "PIKS_SWEEP_BATCH must be 1, 2 or 4");
-    c->batch = v;
-  }
   if (const char* b = std::getenv("PIKS_XCD_REMAP")) c->xcd_remap = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_SWEEP_BLOCK")) {
+    const int v = std::atoi(b);
+    if (v != 64 && v != 128 && v != 256 && v != 512 && v != 1024)
+      return fail(PIKS_E_ARG, "PIKS_SWEEP_BLOCK must be 64, 128, 256, 512 or 1024");
+    c->sweep_block = v;
+  }
   c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
-  const size_t nf = (size_t)g.Ntotal * 8 * c->d.Nc8;
+  c->d.ostride = (g.nstore + 1) * (int64_t)c->d.Nc8;   // + the zero slot
+  const size_t nf = (size_t)c->d.ostride * 8;
   c->n_f = nf;
   c->n_fw = (size_t)std::max(g.ncorner, 1) * 8 * c->d.Nc8;
   c->n_buf = (size_t)g.Ny * g.Nz * 8 * c->d.Nc8;
@@ -772,7 +765,7 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   size_t free_b = 0, total_b = 0;
   HIP_TRY(hipMemGetInfo(&free_b, &total_b));
   const size_t need = nf * (sizeof(double) + (refined ? sizeof(float) : 0)) +
-                      (size_t)g.Ntotal * (32 * 8 * c->nchunk + 4 * 8 + 16) + g.hp_task.size() * 8;
+                      (size_t)g.Ntotal * (32 * 8 + 4 * 8 + 16 + 16) + g.hp_task.size() * 4;
   if (need > free_b)
     return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
                                 std::to_string(free_b >> 20) + " MiB free");
@@ -788,11 +781,8 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
       return rc;
     c->phase = PIKS_PREC_FP32_REFINED;
   }
-  if ((rc = dalloc(*c, &c->rho, (size_t)g.Ntotal))) return rc;
-  if ((rc = dalloc(*c, &c->ux, (size_t)g.Ntotal))) return rc;
-  if ((rc = dalloc(*c, &c->uy, (size_t)g.Ntotal))) return rc;
-  if ((rc = dalloc(*c, &c->uz, (size_t)g.Ntotal))) return rc;
-  if ((rc = dalloc(*c, &c->part, (size_t)g.Ntotal * 32 * c->nchunk))) return rc;
+  if ((rc = dalloc(*c, &c->mom, (size_t)g.Ntotal))) return rc;
+  if ((rc = dalloc(*c, &c->part, (size_t)g.Ntotal * 32))) return rc;
   if ((rc = dalloc(*c, &c->plane, (size_t)g.Ny * g.Nz))) return rc;
   if ((rc = upload(*c, &c->dcx, c->cx)) || (rc = upload(*c, &c->dcy, c->cy)) ||
       (rc = upload(*c, &c->dcz, c->cz)) || (rc = upload(*c, &c->dw, c->w)))
@@ -810,6 +800,10 @@ int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) 
   all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
   if ((rc = upload(*c, &c->fluid_all, all))) return rc;
   if ((rc = upload(*c, &c->hp_task, g.hp_task))) return rc;
+  for (int F = 0; F < 4; ++F) {
+    if ((rc = upload(*c, &c->pos[F], g.pos[F]))) return rc;
+    c->d.pos[F] = c->pos[F];
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   *out = c.release();
   return 0;
@@ -835,9 +829,8 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
   const Dims d = c.d;
   SweepArgs<T> sa{};
   sa.f = st.f; sa.lab = c.lab;
-  sa.nchunk = c.nchunk;
   sa.xcd_remap = c.xcd_remap;
-  sa.rho = c.rho; sa.ux = c.ux; sa.uy = c.uy; sa.uz = c.uz;
+  sa.mom = c.mom;
   sa.part = c.exact ? nullptr : c.part;
   sa.wid = c.wid; sa.views = c.views; sa.wall_corner = c.wall_corner; sa.fw = st.fw;
   sa.cx = c.dcx; sa.cy = c.dcy; sa.cz = c.dcz; sa.w = c.dw;
@@ -857,11 +850,15 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
     for (int h = 0; h < g.nhp; ++h) {
       sa.ntask = g.hp_off[h + 1] - g.hp_off[h];
       if (sa.ntask == 0) continue;
-      sa.tasks = c.hp_task + g.hp_off[h];
-      const int64_t waves = (sa.ntask * c.nchunk + c.batch - 1) / c.batch, wpg = kSweepThreads / kWave;
-      const int64_t blocks = ((waves + wpg - 1) / wpg + 7) / 8 * 8;
-      auto kern = c.batch == 1 ? k_sweep<T, 1> : c.batch == 2 ? k_sweep<T, 2> : k_sweep<T, 4>;
-      hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kSweepThreads), 0, c.stream, sa);
+      sa.tasks = reinterpret_cast<const TaskRec*>(c.hp_task) + g.hp_off[h];
+      const int64_t wpg = c.sweep_block / kWave;
+      const int64_t blocks = ((sa.ntask + wpg - 1) / wpg + 7) / 8 * 8;
+      auto kern = c.sweep_block == 64    ? k_sweep<T, 64>
+                  : c.sweep_block == 128 ? k_sweep<T, 128>
+                  : c.sweep_block == 512 ? k_sweep<T, 512>
+                  : c.sweep_block == 1024 ? k_sweep<T, 1024>
+                                          : k_sweep<T, 256>;
+      hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(c.sweep_block), 0, c.stream, sa);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
@@ -896,17 +893,16 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
     if (c.exact) {
       const int64_t n = (int64_t)g.fluid_interior.size() + (int64_t)g.fluid_boundary.size();
       hipLaunchKernelGGL(k_moments_serial<T>, dim3(grid_for(n, 64)), dim3(64), 0, c.stream,
-                         (const T*)st.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, c.rho, c.ux,
-                         c.uy, c.uz);
+                         (const T*)st.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, c.mom);
     } else {
       const int64_t ni = (int64_t)g.fluid_interior.size(), nbd = (int64_t)g.fluid_boundary.size();
       if (ni > 0)
         hipLaunchKernelGGL(k_moments_partials, dim3(grid_for(ni, 256)), dim3(256), 0, c.stream,
-                           (const double*)c.part, c.nchunk, c.fluid_int, ni, c.rho, c.ux, c.uy, c.uz);
+                           (const double*)c.part, c.fluid_int, ni, c.mom);
       if (nbd > 0)
         hipLaunchKernelGGL(k_moments_direct<T>, dim3((unsigned)((nbd + wpb - 1) / wpb)),
                            dim3(wpb * kWave), 0, c.stream, (const T*)st.f, c.fluid_bnd, nbd,
-                           c.dcx, c.dcy, c.dcz, d, c.rho, c.ux, c.uy, c.uz);
+                           c.dcx, c.dcy, c.dcz, d, c.mom);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 3], c.stream));
   }
@@ -962,7 +958,7 @@ int mass_flow(Ctx& c, double* m) {
   const Dims d = c.d;
   HIP_TRY(hipSetDevice(c.device));
   hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)g.Ny * g.Nz, 256)), dim3(256), 0, c.stream,
-                     (const double*)c.ux, c.plane, d, 2);
+                     (const Mom*)c.mom, c.plane, d, 2);
   std::vector<double> u((size_t)g.Ny * g.Nz);
   HIP_TRY(hipMemcpyAsync(u.data(), c.plane, u.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1034,13 +1030,15 @@ int piks_get_macro(void* ctx, double* rho, double* ux, double* uy, double* uz) {
   if (!ctx) return fail(PIKS_E_ARG, "null context");
   Ctx& c = *static_cast<Ctx*>(ctx);
   HIP_TRY(hipSetDevice(c.device));
-  double* dst[4] = {rho, ux, uy, uz};
-  double* src[4] = {c.rho, c.ux, c.uy, c.uz};
-  for (int q = 0; q < 4; ++q)
-    if (dst[q])
-      HIP_TRY(hipMemcpyAsync(dst[q], src[q], (size_t)c.g.Ntotal * sizeof(double), hipMemcpyDeviceToHost,
-                             c.stream));
+  std::vector<Mom> h((size_t)c.g.Ntotal);
+  HIP_TRY(hipMemcpyAsync(h.data(), c.mom, h.size() * sizeof(Mom), hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
+  for (size_t q = 0; q < h.size(); ++q) {
+    if (rho) rho[q] = h[q].r;
+    if (ux) ux[q] = h[q].x;
+    if (uy) uy[q] = h[q].y;
+    if (uz) uz[q] = h[q].z;
+  }
   return 0;
 }
 
@@ -1049,7 +1047,7 @@ int piks_get_f(void* ctx, double* f) {
   Ctx& c = *static_cast<Ctx*>(ctx);
   HIP_TRY(hipSetDevice(c.device));
   const size_t nc8 = c.d.Nc8, nt = (size_t)c.g.Ntotal;
-  std::vector<double> h(nt * 8 * nc8);
+  std::vector<double> h(c.n_f);
   if (c.phase == PIKS_PREC_FP32_REFINED) {
     std::vector<float> hf(h.size());
     HIP_TRY(hipMemcpyAsync(hf.data(), c.s32.f, hf.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
@@ -1059,10 +1057,14 @@ int piks_get_f(void* ctx, double* f) {
     HIP_TRY(hipMemcpyAsync(h.data(), c.s64.f, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
   }
-  // device [o][cell][l] -> Fortran f(cell, l, o)
-  for (size_t o = 0; o < 8; ++o)
-    for (size_t cell = 0; cell < nt; ++cell)
-      for (size_t l = 0; l < nc8; ++l) f[(o * nc8 + l) * nt + cell] = h[(o * nt + cell) * nc8 + l];
+  // device [o][storage index][l] -> Fortran f(cell, l, o)
+  for (size_t o = 0; o < 8; ++o) {
+    const std::vector<int32_t>& pos = c.g.pos[ofam((int)o)];
+    for (size_t cell = 0; cell < nt; ++cell) {
+      const double* src = &h[o * (size_t)c.d.ostride + (size_t)pos[cell] * nc8];
+      for (size_t l = 0; l < nc8; ++l) f[(o * nc8 + l) * nt + cell] = src[l];
+    }
+  }
   return 0;
 }
 

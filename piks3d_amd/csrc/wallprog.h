@@ -30,6 +30,10 @@ enum Label : int {
 PIKS_HD constexpr int osx(int o) { return (o == 1 || o == 2 || o == 5 || o == 6) ? -1 : 1; }
 PIKS_HD constexpr int osy(int o) { return (o == 2 || o == 3 || o == 6 || o == 7) ? -1 : 1; }
 PIKS_HD constexpr int osz(int o) { return o >= 4 ? -1 : 1; }
+// Octants whose velocity signs are equal or opposite share a storage order
+// (family 0: octants 0, 6; 1: 1, 7; 2: 2, 4; 3: 3, 5); the family's signs are
+// those of octant `family`.
+PIKS_HD constexpr int ofam(int o) { return o < 4 ? o : (o == 4 ? 2 : o == 5 ? 3 : o == 6 ? 0 : 1); }
 
 PIKS_HD constexpr bool is_wall(int lab) { return lab >= 20; }
 PIKS_HD constexpr bool is_corner3(int lab) { return lab >= 60; }
