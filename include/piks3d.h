@@ -40,7 +40,7 @@ extern "C" {
 #define PIKS_E_STATE (-7)    /* call not valid in this state */
 
 #define PIKS_PREC_FP64 0       /* f stored and swept in double */
-#define PIKS_PREC_FP32_REFINED 1 /* f32 sweeps inside an f64 defect correction */
+#define PIKS_PREC_FP32_REFINED 1 /* f stored in FP32 for the bulk of the iterations, FP64 to finish */
 
 /* para.in contents (src/parameters.F90:78-82), after defaults.  nx is the
  * image width before the fluid layers are added (src/parameters.F90:121). */
@@ -147,6 +147,57 @@ int piks_destroy(void* ctx);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* piks_last_error(void);
+
+/* ------------------------------------------------------------------ request server
+ * Continuous batching of independent solves on one GPU.  No counterpart in
+ * the reference, which runs one solve per program (src/main.F90); each
+ * request runs the reference's loop (src/main.F90:95-121): iterations up to
+ * the next check, chkConverge, stop at error <= eps or max_step.  Requests
+ * are preprocessed on host threads, admitted in arrival order whenever a
+ * slot is free and their device footprint fits the memory budget, and run
+ * concurrently, each on its own stream with its iteration captured as a HIP
+ * graph; a finished request releases its slot and memory to the next one.
+ * Results are bit-for-bit those of piks_create + piks_iterate +
+ * piks_chk_converge on the same request.  All calls are thread-safe. */
+typedef struct piks_serve_params {
+    int32_t device;            /* HIP device ordinal */
+    int32_t max_slots;         /* solves on the device at once (<= 64; 0: 8) */
+    int32_t prep_threads;      /* host preprocessing threads (0: 4) */
+    int32_t use_graphs;        /* 1: each iteration is one HIP graph launch */
+    int64_t memory_budget;     /* device bytes for admitted solves (0: 90% of free memory) */
+    int32_t reserved[4];
+} piks_serve_params;
+
+#define PIKS_REQ_QUEUED 0      /* waiting for preprocessing or admission */
+#define PIKS_REQ_RUNNING 1
+#define PIKS_REQ_DONE 2
+#define PIKS_REQ_FAILED 3
+
+typedef struct piks_serve_result {
+    int32_t status;            /* PIKS_REQ_* */
+    int32_t error_code;        /* PIKS_E_* when FAILED */
+    int32_t steps;             /* iterations run */
+    int32_t steps32;           /* of which in FP32 (PIKS_PREC_FP32_REFINED requests) */
+    int32_t converged;         /* 1: error <= eps was reached */
+    int32_t reserved0;
+    double  permeability;      /* last check's K (src/output.F90:155) */
+    double  mass;
+    double  error;
+    double  queue_ms;          /* submit -> admitted (includes host preprocessing) */
+    double  run_ms;            /* admitted -> done */
+    double  total_ms;          /* submit -> done */
+    int64_t device_bytes;      /* device memory the request held */
+} piks_serve_result;
+
+int piks_serve_create(const piks_serve_params* sp, void** server_out);
+/* Queue one solve; p and the image as for piks_create.  refine_switch is the
+ * FP32 -> FP64 switch level of FP32_REFINED requests (<= 0: 1e-6). */
+int piks_serve_submit(void* server, const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
+                      int32_t max_step, double eps, double refine_switch, int64_t* ticket_out);
+int piks_serve_poll(void* server, int64_t ticket, piks_serve_result* out);   /* no wait */
+int piks_serve_wait(void* server, int64_t ticket, piks_serve_result* out);   /* until done or failed */
+int piks_serve_drain(void* server);                                          /* until all are */
+int piks_serve_destroy(void* server);
 
 #ifdef __cplusplus
 }

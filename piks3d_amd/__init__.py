@@ -57,6 +57,30 @@ class PiksInfo(ctypes.Structure):
     ]
 
 
+class ServeParams(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32), ("max_slots", ctypes.c_int32),
+        ("prep_threads", ctypes.c_int32), ("use_graphs", ctypes.c_int32),
+        ("memory_budget", ctypes.c_int64), ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("error_code", ctypes.c_int32),
+        ("steps", ctypes.c_int32), ("steps32", ctypes.c_int32),
+        ("converged", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("permeability", ctypes.c_double), ("mass", ctypes.c_double), ("error", ctypes.c_double),
+        ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
 # every entry point include/piks3d.h declares, with its ctypes signature
 _API = {
     "piks_create": (ctypes.c_int, [ctypes.POINTER(PiksParams), ctypes.c_void_p, ctypes.c_int64,
@@ -75,6 +99,14 @@ _API = {
     "piks_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "piks_last_error": (ctypes.c_char_p, []),
+    "piks_serve_create": (ctypes.c_int, [ctypes.POINTER(ServeParams), ctypes.POINTER(ctypes.c_void_p)]),
+    "piks_serve_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PiksParams), ctypes.c_void_p,
+                                         ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_int64)]),
+    "piks_serve_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ServeResult)]),
+    "piks_serve_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ServeResult)]),
+    "piks_serve_drain": (ctypes.c_int, [ctypes.c_void_p]),
+    "piks_serve_destroy": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None
@@ -247,7 +279,8 @@ class Solver:
             step += n
             if refined:
                 steps32 += n
-            if step % chk == 0:
+            checked = step % chk == 0
+            if checked:
                 m, perm, error = self.chk_converge()
                 self.history.append((step, m, perm, error))
                 if callback:
@@ -255,14 +288,15 @@ class Solver:
             if refined:
                 # FP32 has converged as far as it can once the error is below
                 # the switch level or has not improved for three checks
-                if error < best:
-                    best, stalled = error, 0
-                else:
-                    stalled += 1
-                if error <= max(tau, eps) or stalled >= 3:
-                    self.set_precision("fp64")
-                    refined = False
-                    error = 1.0
+                if checked:
+                    if error < best:
+                        best, stalled = error, 0
+                    else:
+                        stalled += 1
+                    if error <= max(tau, eps) or stalled >= 3:
+                        self.set_precision("fp64")
+                        refined = False
+                        error = 1.0
                 continue
             if error <= eps:
                 if step < chk:
@@ -271,3 +305,63 @@ class Solver:
                 break
         return dict(steps=step, steps32=steps32, permeability=perm, mass=self.mass, error=error,
                     history=list(self.history))
+
+
+class Server:
+    """Continuous-batching request server on one GPU (piks_serve_* in
+    include/piks3d.h): submit returns at once; requests are preprocessed on
+    host threads, admitted as slots and device memory free up, and run
+    concurrently.  Each result is the same as a Solver running the same
+    request alone."""
+
+    def __init__(self, device=0, max_slots=8, prep_threads=4, use_graphs=True, memory_budget=0):
+        sp = ServeParams()
+        sp.device, sp.max_slots, sp.prep_threads = device, max_slots, prep_threads
+        sp.use_graphs = int(bool(use_graphs))
+        sp.memory_budget = int(memory_budget)
+        h = ctypes.c_void_p()
+        _check(lib().piks_serve_create(ctypes.byref(sp), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, params: Params, image, max_step=None, eps=None, precision=None, refine_switch=None):
+        """Queue one solve (the reference's loop with params' chkConvergeStep);
+        returns its ticket."""
+        raw = np.ascontiguousarray(np.asarray(image, dtype=np.uint8).ravel())
+        cp = to_cparams(params, precision)
+        t = ctypes.c_int64()
+        _check(lib().piks_serve_submit(
+            self._h, ctypes.byref(cp), raw.ctypes.data, raw.size,
+            int(params.max_step if max_step is None else max_step),
+            float(params.eps if eps is None else eps),
+            float(params.refine_switch if refine_switch is None else refine_switch), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        r = ServeResult()
+        _check(lib().piks_serve_poll(self._h, ticket, ctypes.byref(r)))
+        return r.as_dict()
+
+    def wait(self, ticket):
+        r = ServeResult()
+        _check(lib().piks_serve_wait(self._h, ticket, ctypes.byref(r)))
+        return r.as_dict()
+
+    def drain(self):
+        _check(lib().piks_serve_drain(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().piks_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -47,7 +47,7 @@ def build_lib(verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for src in ("piks3d.hip", "geometry.cpp"):
+    for src in ("piks3d.hip", "serve.hip", "geometry.cpp"):
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)

@@ -1,0 +1,127 @@
+// context.h -- device state of one solve and the internal API that the C ABI
+// (piks3d.hip) and the request server (serve.hip) share.
+//
+// A context is built in two phases: ctx_create_host does the host work (the
+// velocity grid and the image preprocessing, no HIP call, safe on any
+// thread), ctx_create_device allocates and uploads on one stream.  After that
+// every operation is enqueued on the context's stream; only the functions
+// that say so synchronise.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/piks3d.h"
+#include "geometry.h"
+
+namespace piks {
+
+struct Dims {
+  int Nx, Ny, Nz, Nxt;
+  int64_t Nxyt, Ntotal;
+  int Nc8;
+  // storage of f: octant o keeps its cells in family ofam(o)'s order, pos[F]
+  // maps a cell to that order (unstored cells to the zero slot); ostride =
+  // elements per octant
+  const int32_t* pos[4];
+  int64_t ostride;
+};
+
+// Moments of one cell (src/flow.F90:72 keeps Rho, Ux, Uy, Uz as four
+// arrays); interleaved so that a sweep task loads them with one scalar load.
+struct alignas(32) Mom {
+  double r, x, y, z;
+};
+
+// Sweep task record (see Geometry::hp_task), loaded with one scalar load.
+struct alignas(32) TaskRec {
+  int32_t w[8];
+};
+
+// Device state that carries from one iteration to the next, in one precision.
+template <typename T>
+struct State {
+  T* f = nullptr;        // [octant][storage index][velocity]
+  T* fw = nullptr;       // [corner][octant][velocity]
+  T* buf_e = nullptr;    // x = Nx plane after the previous sweep (ghost x = 0 next time)
+  T* buf_w = nullptr;    // x = 1 plane after the previous sweep (ghost x = Nx+1 next time)
+  T* snap = nullptr;     // walls read by earlier coupled walls, before the wall pass
+};
+
+struct Ctx {
+  piks_params prm{};
+  Geometry g;
+  Dims d{};
+  int nc_fund = 0;
+  std::vector<double> cx, cy, cz, w;
+  double mu = 0, diff_flux = 0, mass = 1.0;
+  int exact = 0;               // serial reductions in the reference's order
+  hipStream_t stream = nullptr;
+  bool own_stream = true;      // false: the stream belongs to the caller
+  bool stream_ordered = false; // allocations with hipMallocAsync / hipFreeAsync on `stream`
+  int device = 0;
+  int64_t bytes = 0;
+  // device state: FP64 always; FP32 too for PIKS_PREC_FP32_REFINED contexts
+  State<double> s64;
+  State<float> s32;
+  int phase = PIKS_PREC_FP64;  // precision the next iterations run in
+  size_t n_f = 0, n_fw = 0, n_buf = 0, n_snap = 0;
+  Mom* mom = nullptr;
+  double* part = nullptr;
+  double* plane = nullptr;
+  double *dcx = nullptr, *dcy = nullptr, *dcz = nullptr, *dw = nullptr;
+  uint8_t* lab = nullptr;
+  int32_t* wid = nullptr;
+  uint64_t* views = nullptr;
+  int32_t *wall_cell = nullptr, *wall_corner = nullptr;
+  uint8_t* wall_label = nullptr;
+  double *e1 = nullptr, *e2 = nullptr;
+  int32_t *uncoupled = nullptr, *coupled = nullptr, *coupled_snap = nullptr, *snap_cell = nullptr;
+  int32_t* commit_walls = nullptr;
+  int32_t *fluid_int = nullptr, *fluid_bnd = nullptr, *fluid_all = nullptr;
+  // sweep schedule: tasks of each oriented hyperplane
+  bool xcd_remap = true;       // contiguous task ranges per XCD
+  int sweep_block = 64;        // threads per sweep workgroup (one wave: a task's
+                               // workgroup ends with its wave, measured fastest)
+  int32_t* hp_task = nullptr;
+  int32_t* pos[4] = {nullptr, nullptr, nullptr, nullptr};
+  // one iteration captured as a HIP graph, per precision (used by ctx_enqueue
+  // when use_graph is set)
+  bool use_graph = false;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // timing: four events per step of the last timed call
+  std::vector<hipEvent_t> ev;
+  double last_ms[4] = {0, 0, 0, 0};
+  std::vector<void*> allocs;
+};
+
+// Records the calling thread's last error message and returns code.
+int set_error(int code, const std::string& msg);
+
+// Host phase of set-up: parameter checks, velocity grid, image preprocessing.
+int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out);
+// Device memory ctx_create_device will allocate (bytes).
+size_t ctx_device_bytes(const Ctx& c);
+// Device phase: allocate, clear and upload.  stream == nullptr: the context
+// creates its own stream and uses hipMalloc; otherwise it works on `stream`
+// with stream-ordered allocations.  Does not synchronise when given a stream.
+int ctx_create_device(Ctx& c, hipStream_t stream);
+// Enqueue nsteps iterations (no synchronisation, no timing).
+int ctx_enqueue(Ctx& c, int nsteps);
+// Enqueue the copy of Ux on the plane x = 2 (Ny*Nz doubles, j fastest) into
+// host memory (pinned, so the copy is asynchronous).
+int ctx_enqueue_plane(Ctx& c, double* host_plane);
+// Mass flow through x = 2 from that plane, in the reference's summation order
+// (massLocal of src/output.F90:80-142).
+double ctx_mass(const Ctx& c, const double* plane);
+// chkConverge's update of the stored mass (src/output.F90:151-155):
+// out = {mass, permeability, error}.
+void ctx_check(Ctx& c, double m, double out[3]);
+// Switch the carried state's precision (synchronises the stream).
+int ctx_set_phase(Ctx& c, int phase);
+// Free everything (stream-ordered frees on a caller's stream do not block).
+void ctx_destroy(Ctx* c);
+
+}  // namespace piks

@@ -27,20 +27,27 @@
 #include <vector>
 
 #include "../../include/piks3d.h"
+#include "context.h"
 #include "gauss_hermite_tables.h"
 #include "geometry.h"
 #include "wallprog.h"
 
 using namespace piks;
 
-namespace {
+namespace piks {
 
 thread_local std::string g_err;
 
-int fail(int code, const std::string& msg) {
+int set_error(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+}  // namespace piks
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 #define HIP_TRY(expr)                                                                  \
   do {                                                                                 \
@@ -65,28 +72,6 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
   int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
   return ((int64_t)hi << 32) | (uint32_t)lo;
 }
-
-struct Dims {
-  int Nx, Ny, Nz, Nxt;
-  int64_t Nxyt, Ntotal;
-  int Nc8;
-  // storage of f: octant o keeps its cells in family ofam(o)'s order, pos[F]
-  // maps a cell to that order (unstored cells to the zero slot); ostride =
-  // elements per octant
-  const int32_t* pos[4];
-  int64_t ostride;
-};
-
-// Moments of one cell (src/flow.F90:72 keeps Rho, Ux, Uy, Uz as four
-// arrays); interleaved so that a task loads them with one scalar load.
-struct alignas(32) Mom {
-  double r, x, y, z;
-};
-
-// Sweep task record (see Geometry::hp_task), loaded with one scalar load.
-struct alignas(32) TaskRec {
-  int32_t w[8];
-};
 
 // Read-only data of a kernel loaded through the scalar cache: the constant
 // address space tells the compiler that no store of the kernel can clobber it.
@@ -600,22 +585,6 @@ __global__ void k_gather_plane(const Mom* mom, double* out, Dims d, int column) 
 // ------------------------------------------------------------------ context
 
 template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-};
-
-// Device state that carries from one iteration to the next, in one precision.
-template <typename T>
-struct State {
-  T* f = nullptr;        // [octant][cell][velocity]
-  T* fw = nullptr;       // [corner][octant][velocity]
-  T* buf_e = nullptr;    // x = Nx plane after the previous sweep (ghost x = 0 next time)
-  T* buf_w = nullptr;    // x = 1 plane after the previous sweep (ghost x = Nx+1 next time)
-  T* snap = nullptr;     // walls read by earlier coupled walls, before the wall pass
-};
-
-template <typename T>
 __global__ void k_convert(T* dst, const double* src, size_t n) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
     dst[t] = (T)src[t];
@@ -625,57 +594,16 @@ __global__ void k_widen(double* dst, const float* src, size_t n) {
     dst[t] = (double)src[t];
 }
 
-struct Ctx {
-  piks_params prm{};
-  Geometry g;
-  Dims d{};
-  int nc_fund = 0;
-  std::vector<double> cx, cy, cz, w;
-  double mu = 0, diff_flux = 0, mass = 1.0;
-  int exact = 0;         // serial reductions in the reference's order
-  hipStream_t stream = nullptr;
-  int device = 0;
-  int64_t bytes = 0;
-  // device state: FP64 always; FP32 too for PIKS_PREC_FP32_REFINED contexts
-  State<double> s64;
-  State<float> s32;
-  int phase = PIKS_PREC_FP64;   // precision the next iterations run in
-  size_t n_f = 0, n_fw = 0, n_buf = 0, n_snap = 0;
-  Mom* mom = nullptr;
-  double* part = nullptr;
-  double* plane = nullptr;
-  double *dcx = nullptr, *dcy = nullptr, *dcz = nullptr, *dw = nullptr;
-  uint8_t* lab = nullptr;
-  int32_t* wid = nullptr;
-  uint64_t* views = nullptr;
-  int32_t *wall_cell = nullptr, *wall_corner = nullptr;
-  uint8_t* wall_label = nullptr;
-  double *e1 = nullptr, *e2 = nullptr;
-  int32_t *uncoupled = nullptr, *coupled = nullptr, *coupled_snap = nullptr, *snap_cell = nullptr;
-  int32_t *commit_walls = nullptr;
-  int32_t *fluid_int = nullptr, *fluid_bnd = nullptr, *fluid_all = nullptr;
-  // sweep schedule: tasks of each oriented hyperplane
-  bool xcd_remap = true;       // contiguous task ranges per XCD
-  int sweep_block = 64;        // threads per sweep workgroup (one wave: a task's
-                               // workgroup ends with its wave, measured fastest)
-  int32_t* hp_task = nullptr;
-  int32_t* pos[4] = {nullptr, nullptr, nullptr, nullptr};
-  // timing: four events per step of the last piks_iterate call
-  std::vector<hipEvent_t> ev;
-  double last_ms[4] = {0, 0, 0, 0};
-  std::vector<void*> allocs;
-};
-
 template <typename T>
 int dalloc(Ctx& c, T** p, size_t n) {
   if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  const size_t bytes = n * sizeof(T);
+  hipError_t e = c.stream_ordered ? hipMallocAsync((void**)p, bytes, c.stream) : hipMalloc((void**)p, bytes);
   if (e != hipSuccess)
-    return fail(PIKS_E_MEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes: " +
-                                hipGetErrorString(e));
+    return fail(PIKS_E_MEM, "device allocation of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
   c.allocs.push_back(*p);
-  c.bytes += (int64_t)(n * sizeof(T));
-  HIP_TRY(hipMemsetAsync(*p, 0, n * sizeof(T), c.stream));
+  c.bytes += (int64_t)bytes;
+  HIP_TRY(hipMemsetAsync(*p, 0, bytes, c.stream));
   return 0;
 }
 
@@ -689,16 +617,6 @@ int upload(Ctx& c, T** p, const std::vector<T>& v) {
 int grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
-}
-
-void free_ctx(Ctx* c) {
-  if (!c) return;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  for (void* p : c->allocs) hipFree(p);
-  for (auto& e : c->ev) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
-  delete c;
 }
 
 int setup_velocity(Ctx& c) {
@@ -726,89 +644,6 @@ int setup_velocity(Ctx& c) {
   return 0;
 }
 
-int create(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) {
-  if (!p || !raw || !out) return fail(PIKS_E_ARG, "null argument");
-  if (p->precision != PIKS_PREC_FP64 && p->precision != PIKS_PREC_FP32_REFINED)
-    return fail(PIKS_E_ARG, "unknown precision mode " + std::to_string(p->precision));
-  if (!(p->kn > 0.0)) return fail(PIKS_E_ARG, "Kn must be positive");
-  if (p->chk_converge_step < 1) return fail(PIKS_E_ARG, "chkConvergeStep must be >= 1");
-  std::unique_ptr<Ctx, void (*)(Ctx*)> c(new Ctx(), free_ctx);
-  c->prm = *p;
-  c->exact = p->exact_order != 0;
-  c->device = p->device;
-  HIP_TRY(hipSetDevice(c->device));
-  if (int rc = setup_velocity(*c)) return rc;
-  std::string err;
-  GeomInput gi{p->nx, p->ny, p->nz, p->fluid_layer, p->wall_ext_order};
-  if (p->wall_ext_order < 1 || p->wall_ext_order > 3)
-    std::fprintf(stderr, " Error: wallExtOrder wroond shoud be [1|2|3]\n");  // src/physicalGrid.F90:947
-  if (int rc = build_geometry(gi, raw, nbytes, c->g, err)) return fail(rc, err);
-  const Geometry& g = c->g;
-  c->d = Dims{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, g.Ntotal, (int)c->cx.size()};
-  if (const char* b = std::getenv("PIKS_XCD_REMAP")) c->xcd_remap = std::atoi(b) != 0;
-  if (const char* b = std::getenv("PIKS_SWEEP_BLOCK")) {
-    const int v = std::atoi(b);
-    if (v != 64 && v != 128 && v != 256 && v != 512 && v != 1024)
-      return fail(PIKS_E_ARG, "PIKS_SWEEP_BLOCK must be 64, 128, 256, 512 or 1024");
-    c->sweep_block = v;
-  }
-  c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
-  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-
-  c->d.ostride = (g.nstore + 1) * (int64_t)c->d.Nc8;   // + the zero slot
-  const size_t nf = (size_t)c->d.ostride * 8;
-  c->n_f = nf;
-  c->n_fw = (size_t)std::max(g.ncorner, 1) * 8 * c->d.Nc8;
-  c->n_buf = (size_t)g.Ny * g.Nz * 8 * c->d.Nc8;
-  c->n_snap = std::max<size_t>(g.snap_cell.size(), 1) * 8 * c->d.Nc8;
-  const bool refined = p->precision == PIKS_PREC_FP32_REFINED;
-  size_t free_b = 0, total_b = 0;
-  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-  const size_t need = nf * (sizeof(double) + (refined ? sizeof(float) : 0)) +
-                      (size_t)g.Ntotal * (32 * 8 + 4 * 8 + 16 + 16) + g.hp_task.size() * 4;
-  if (need > free_b)
-    return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
-                                std::to_string(free_b >> 20) + " MiB free");
-  int rc = 0;
-  if ((rc = dalloc(*c, &c->s64.f, nf)) || (rc = dalloc(*c, &c->s64.fw, c->n_fw)) ||
-      (rc = dalloc(*c, &c->s64.buf_e, c->n_buf)) || (rc = dalloc(*c, &c->s64.buf_w, c->n_buf)) ||
-      (rc = dalloc(*c, &c->s64.snap, c->n_snap)))
-    return rc;
-  if (refined) {
-    if ((rc = dalloc(*c, &c->s32.f, nf)) || (rc = dalloc(*c, &c->s32.fw, c->n_fw)) ||
-        (rc = dalloc(*c, &c->s32.buf_e, c->n_buf)) || (rc = dalloc(*c, &c->s32.buf_w, c->n_buf)) ||
-        (rc = dalloc(*c, &c->s32.snap, c->n_snap)))
-      return rc;
-    c->phase = PIKS_PREC_FP32_REFINED;
-  }
-  if ((rc = dalloc(*c, &c->mom, (size_t)g.Ntotal))) return rc;
-  if ((rc = dalloc(*c, &c->part, (size_t)g.Ntotal * 32))) return rc;
-  if ((rc = dalloc(*c, &c->plane, (size_t)g.Ny * g.Nz))) return rc;
-  if ((rc = upload(*c, &c->dcx, c->cx)) || (rc = upload(*c, &c->dcy, c->cy)) ||
-      (rc = upload(*c, &c->dcz, c->cz)) || (rc = upload(*c, &c->dw, c->w)))
-    return rc;
-  if ((rc = upload(*c, &c->lab, g.lab)) ||
-      (rc = upload(*c, &c->wid, g.wid)) || (rc = upload(*c, &c->views, g.wall_views)) ||
-      (rc = upload(*c, &c->wall_cell, g.wall_cell)) || (rc = upload(*c, &c->wall_corner, g.wall_corner)) ||
-      (rc = upload(*c, &c->wall_label, g.wall_label)) || (rc = upload(*c, &c->e1, g.wall_e1)) ||
-      (rc = upload(*c, &c->e2, g.wall_e2)) || (rc = upload(*c, &c->uncoupled, g.uncoupled)) ||
-      (rc = upload(*c, &c->coupled, g.coupled)) || (rc = upload(*c, &c->coupled_snap, g.coupled_snap)) ||
-      (rc = upload(*c, &c->snap_cell, g.snap_cell)) || (rc = upload(*c, &c->commit_walls, g.commit_walls)) ||
-      (rc = upload(*c, &c->fluid_int, g.fluid_interior)) || (rc = upload(*c, &c->fluid_bnd, g.fluid_boundary)))
-    return rc;
-  std::vector<int32_t> all(g.fluid_interior);
-  all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
-  if ((rc = upload(*c, &c->fluid_all, all))) return rc;
-  if ((rc = upload(*c, &c->hp_task, g.hp_task))) return rc;
-  for (int F = 0; F < 4; ++F) {
-    if ((rc = upload(*c, &c->pos[F], g.pos[F]))) return rc;
-    c->d.pos[F] = c->pos[F];
-  }
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  *out = c.release();
-  return 0;
-}
-
 template <typename T>
 WallArgs<T> wall_args(Ctx& c, State<T>& st, const int32_t* list, int n) {
   WallArgs<T> a{};
@@ -822,9 +657,11 @@ WallArgs<T> wall_args(Ctx& c, State<T>& st, const int32_t* list, int n) {
   return a;
 }
 
+// Enqueue nsteps iterations on the context's stream.  With `timed`, four
+// events per step (at most kMaxTimedSteps) bracket the sweep, boundary and
+// moment phases; the caller synchronises and reads them.
 template <typename T>
-int iterate_t(Ctx& c, State<T>& st, int nsteps) {
-  HIP_TRY(hipSetDevice(c.device));
+int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
   const Geometry& g = c.g;
   const Dims d = c.d;
   SweepArgs<T> sa{};
@@ -838,7 +675,7 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
   sa.c0 = 1.5 / g.ds; sa.c1 = 2.0 / g.ds; sa.c2 = -0.5 / g.ds; sa.d1 = 1.0 / g.ds;
   sa.d = d;
   const int wpb = 4;  // waves per block for one-wave-per-item kernels
-  const int timed = std::min(nsteps, kMaxTimedSteps);
+  const int timed = timed_run ? std::min(nsteps, kMaxTimedSteps) : 0;
   while ((int)c.ev.size() < 4 * timed) {
     hipEvent_t e;
     HIP_TRY(hipEventCreate(&e));
@@ -907,7 +744,17 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 3], c.stream));
   }
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// Timed synchronous iterations (piks_iterate).
+int iterate(Ctx& c, int nsteps) {
+  HIP_TRY(hipSetDevice(c.device));
+  int rc = c.phase == PIKS_PREC_FP32_REFINED ? enqueue_t<float>(c, c.s32, nsteps, true)
+                                              : enqueue_t<double>(c, c.s64, nsteps, true);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c.stream));
+  const int timed = std::min(nsteps, kMaxTimedSteps);
   if (timed > 0) {
     double acc[3] = {0, 0, 0};
     for (int st = 0; st < timed; ++st)
@@ -923,13 +770,190 @@ int iterate_t(Ctx& c, State<T>& st, int nsteps) {
   return 0;
 }
 
-int iterate(Ctx& c, int nsteps) {
-  if (c.phase == PIKS_PREC_FP32_REFINED) return iterate_t<float>(c, c.s32, nsteps);
-  return iterate_t<double>(c, c.s64, nsteps);
+}  // namespace
+
+// ------------------------------------------------------------------ context API
+
+namespace piks {
+
+int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ctx** out) {
+  if (!p || !raw || !out) return fail(PIKS_E_ARG, "null argument");
+  *out = nullptr;
+  if (p->precision != PIKS_PREC_FP64 && p->precision != PIKS_PREC_FP32_REFINED)
+    return fail(PIKS_E_ARG, "unknown precision mode " + std::to_string(p->precision));
+  if (!(p->kn > 0.0)) return fail(PIKS_E_ARG, "Kn must be positive");
+  if (p->chk_converge_step < 1) return fail(PIKS_E_ARG, "chkConvergeStep must be >= 1");
+  std::unique_ptr<Ctx> c(new Ctx());
+  c->prm = *p;
+  c->exact = p->exact_order != 0;
+  c->device = p->device;
+  if (int rc = setup_velocity(*c)) return rc;
+  std::string err;
+  GeomInput gi{p->nx, p->ny, p->nz, p->fluid_layer, p->wall_ext_order};
+  if (p->wall_ext_order < 1 || p->wall_ext_order > 3)
+    std::fprintf(stderr, " Error: wallExtOrder wroond shoud be [1|2|3]\n");  // src/physicalGrid.F90:947
+  if (int rc = build_geometry(gi, raw, nbytes, c->g, err)) return fail(rc, err);
+  const Geometry& g = c->g;
+  c->d = Dims{g.Nx, g.Ny, g.Nz, g.Nxt, g.Nxyt, g.Ntotal, (int)c->cx.size(), {}, 0};
+  c->d.ostride = (g.nstore + 1) * (int64_t)c->d.Nc8;   // + the zero slot
+  c->n_f = (size_t)c->d.ostride * 8;
+  c->n_fw = (size_t)std::max(g.ncorner, 1) * 8 * c->d.Nc8;
+  c->n_buf = (size_t)g.Ny * g.Nz * 8 * c->d.Nc8;
+  c->n_snap = std::max<size_t>(g.snap_cell.size(), 1) * 8 * c->d.Nc8;
+  if (const char* b = std::getenv("PIKS_XCD_REMAP")) c->xcd_remap = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_SWEEP_BLOCK")) {
+    const int v = std::atoi(b);
+    if (v != 64 && v != 128 && v != 256 && v != 512 && v != 1024)
+      return fail(PIKS_E_ARG, "PIKS_SWEEP_BLOCK must be 64, 128, 256, 512 or 1024");
+    c->sweep_block = v;
+  }
+  if (const char* b = std::getenv("PIKS_GRAPH")) c->use_graph = std::atoi(b) != 0;
+  c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
+  *out = c.release();
+  return 0;
 }
 
-// Move the carried state between precisions (the moments are FP64 in both).
-int set_phase(Ctx& c, int phase) {
+size_t ctx_device_bytes(const Ctx& c) {
+  const Geometry& g = c.g;
+  const bool refined = c.prm.precision == PIKS_PREC_FP32_REFINED;
+  const size_t state64 = (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(double);
+  const size_t state32 = refined ? (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(float) : 0;
+  const size_t per_cell = (size_t)g.Ntotal * (sizeof(Mom) + 32 * sizeof(double) + 1 + 4 + 4 * 4);
+  const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.uncoupled.size() * 4 +
+                       g.coupled.size() * 4 * 7 + g.snap_cell.size() * 4 + g.commit_walls.size() * 4;
+  const size_t fluid = (size_t)(g.fluid_interior.size() + g.fluid_boundary.size()) * 8;
+  return state64 + state32 + per_cell + walls + fluid + g.hp_task.size() * 4 +
+         (size_t)g.Ny * g.Nz * 8 + 4 * c.cx.size() * 8 + 64 * 256;
+}
+
+int ctx_create_device(Ctx& c, hipStream_t stream) {
+  HIP_TRY(hipSetDevice(c.device));
+  if (stream) {
+    c.stream = stream;
+    c.own_stream = false;
+    c.stream_ordered = true;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.own_stream = true;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t need = ctx_device_bytes(c);
+    if (need > free_b)
+      return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
+                                  std::to_string(free_b >> 20) + " MiB free");
+  }
+  const Geometry& g = c.g;
+  const bool refined = c.prm.precision == PIKS_PREC_FP32_REFINED;
+  int rc = 0;
+  if ((rc = dalloc(c, &c.s64.f, c.n_f)) || (rc = dalloc(c, &c.s64.fw, c.n_fw)) ||
+      (rc = dalloc(c, &c.s64.buf_e, c.n_buf)) || (rc = dalloc(c, &c.s64.buf_w, c.n_buf)) ||
+      (rc = dalloc(c, &c.s64.snap, c.n_snap)))
+    return rc;
+  if (refined) {
+    if ((rc = dalloc(c, &c.s32.f, c.n_f)) || (rc = dalloc(c, &c.s32.fw, c.n_fw)) ||
+        (rc = dalloc(c, &c.s32.buf_e, c.n_buf)) || (rc = dalloc(c, &c.s32.buf_w, c.n_buf)) ||
+        (rc = dalloc(c, &c.s32.snap, c.n_snap)))
+      return rc;
+    c.phase = PIKS_PREC_FP32_REFINED;
+  }
+  if ((rc = dalloc(c, &c.mom, (size_t)g.Ntotal))) return rc;
+  if ((rc = dalloc(c, &c.part, (size_t)g.Ntotal * 32))) return rc;
+  if ((rc = dalloc(c, &c.plane, (size_t)g.Ny * g.Nz))) return rc;
+  if ((rc = upload(c, &c.dcx, c.cx)) || (rc = upload(c, &c.dcy, c.cy)) ||
+      (rc = upload(c, &c.dcz, c.cz)) || (rc = upload(c, &c.dw, c.w)))
+    return rc;
+  if ((rc = upload(c, &c.lab, g.lab)) ||
+      (rc = upload(c, &c.wid, g.wid)) || (rc = upload(c, &c.views, g.wall_views)) ||
+      (rc = upload(c, &c.wall_cell, g.wall_cell)) || (rc = upload(c, &c.wall_corner, g.wall_corner)) ||
+      (rc = upload(c, &c.wall_label, g.wall_label)) || (rc = upload(c, &c.e1, g.wall_e1)) ||
+      (rc = upload(c, &c.e2, g.wall_e2)) || (rc = upload(c, &c.uncoupled, g.uncoupled)) ||
+      (rc = upload(c, &c.coupled, g.coupled)) || (rc = upload(c, &c.coupled_snap, g.coupled_snap)) ||
+      (rc = upload(c, &c.snap_cell, g.snap_cell)) || (rc = upload(c, &c.commit_walls, g.commit_walls)) ||
+      (rc = upload(c, &c.fluid_int, g.fluid_interior)) || (rc = upload(c, &c.fluid_bnd, g.fluid_boundary)))
+    return rc;
+  std::vector<int32_t> all(g.fluid_interior);
+  all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
+  if ((rc = upload(c, &c.fluid_all, all))) return rc;
+  if ((rc = upload(c, &c.hp_task, g.hp_task))) return rc;
+  for (int F = 0; F < 4; ++F) {
+    if ((rc = upload(c, &c.pos[F], g.pos[F]))) return rc;
+    c.d.pos[F] = c.pos[F];
+  }
+  if (c.own_stream) HIP_TRY(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+// Capture one iteration of the current precision as a graph.  The captured
+// work has no memset or allocation node: only kernel launches.
+static int capture_graph(Ctx& c) {
+  const int k = c.phase == PIKS_PREC_FP32_REFINED ? 1 : 0;
+  if (c.graph[k]) return 0;
+  hipGraph_t gr = nullptr;
+  HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  const int rc = k ? enqueue_t<float>(c, c.s32, 1, false) : enqueue_t<double>(c, c.s64, 1, false);
+  const hipError_t e = hipStreamEndCapture(c.stream, &gr);
+  if (rc) {
+    if (gr) hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PIKS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&c.graph[k], gr, nullptr, nullptr, 0);
+  hipGraphDestroy(gr);
+  if (ei != hipSuccess) return fail(PIKS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return 0;
+}
+
+int ctx_enqueue(Ctx& c, int nsteps) {
+  HIP_TRY(hipSetDevice(c.device));
+  if (c.use_graph) {
+    if (int rc = capture_graph(c)) return rc;
+    const int k = c.phase == PIKS_PREC_FP32_REFINED ? 1 : 0;
+    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c.graph[k], c.stream));
+    return 0;
+  }
+  return c.phase == PIKS_PREC_FP32_REFINED ? enqueue_t<float>(c, c.s32, nsteps, false)
+                                            : enqueue_t<double>(c, c.s64, nsteps, false);
+}
+
+int ctx_enqueue_plane(Ctx& c, double* host_plane) {
+  const Geometry& g = c.g;
+  HIP_TRY(hipSetDevice(c.device));
+  hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)g.Ny * g.Nz, 256)), dim3(256), 0, c.stream,
+                     (const Mom*)c.mom, c.plane, c.d, 2);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(host_plane, c.plane, (size_t)g.Ny * g.Nz * sizeof(double), hipMemcpyDeviceToHost,
+                         c.stream));
+  return 0;
+}
+
+double ctx_mass(const Ctx& c, const double* u) {
+  const Geometry& g = c.g;
+  auto U = [&](int j, int k) { return u[(size_t)(j - 1) + (size_t)(k - 1) * g.Ny]; };
+  const double ds = g.ds;
+  const int Ny = g.Ny, Nz = g.Nz;
+  double inner = 0.0, suth = 0.0, noth = 0.0, frnt = 0.0, back = 0.0;
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny - 1; ++j) inner = inner + U(j, k) * ds * ds;
+  for (int k = 2; k <= Nz - 1; ++k) suth = suth + U(1, k) * ds * ds * 0.5;
+  for (int k = 2; k <= Nz - 1; ++k) noth = noth + U(Ny, k) * ds * ds * 0.5;
+  for (int j = 2; j <= Ny - 1; ++j) back = back + U(j, 1) * ds * ds * 0.5;
+  for (int j = 2; j <= Ny - 1; ++j) frnt = frnt + U(j, Nz) * ds * ds * 0.5;
+  const double sb = U(1, 1) * ds * ds * 0.25, sf = U(1, Nz) * ds * ds * 0.25;
+  const double nb = U(Ny, 1) * ds * ds * 0.25, nf = U(Ny, Nz) * ds * ds * 0.25;
+  const double rl = 1.0 / c.prm.ref_l;
+  return (inner + suth + noth + frnt + back + sb + sf + nb + nf) * std::sqrt(1.0 / 2.0) /
+         c.prm.press_drop / (rl * rl);
+}
+
+void ctx_check(Ctx& c, double m, double out[3]) {
+  const double error = std::fabs(1.0 - m / c.mass) / (double)c.prm.chk_converge_step;
+  c.mass = m;
+  out[0] = m;
+  out[1] = m * c.prm.kn * std::sqrt(4.0 / (std::atan(1.0) * 4.0));
+  out[2] = error;
+}
+
+int ctx_set_phase(Ctx& c, int phase) {
   if (phase == c.phase) return 0;
   if (phase == PIKS_PREC_FP32_REFINED && !c.s32.f)
     return fail(PIKS_E_STATE, "context was created without FP32 storage");
@@ -947,40 +971,28 @@ int set_phase(Ctx& c, int phase) {
     hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_w, (const double*)c.s64.buf_w, c.n_buf);
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(c.stream));
+  if (c.own_stream) HIP_TRY(hipStreamSynchronize(c.stream));
   c.phase = phase;
   return 0;
 }
 
-// src/output.F90:80-142 on the host, in the reference's summation order
-int mass_flow(Ctx& c, double* m) {
-  const Geometry& g = c.g;
-  const Dims d = c.d;
-  HIP_TRY(hipSetDevice(c.device));
-  hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)g.Ny * g.Nz, 256)), dim3(256), 0, c.stream,
-                     (const Mom*)c.mom, c.plane, d, 2);
-  std::vector<double> u((size_t)g.Ny * g.Nz);
-  HIP_TRY(hipMemcpyAsync(u.data(), c.plane, u.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_TRY(hipStreamSynchronize(c.stream));
-  auto U = [&](int j, int k) { return u[(size_t)(j - 1) + (size_t)(k - 1) * g.Ny]; };
-  const double ds = g.ds;
-  const int Ny = g.Ny, Nz = g.Nz;
-  double inner = 0.0, suth = 0.0, noth = 0.0, frnt = 0.0, back = 0.0;
-  for (int k = 2; k <= Nz - 1; ++k)
-    for (int j = 2; j <= Ny - 1; ++j) inner = inner + U(j, k) * ds * ds;
-  for (int k = 2; k <= Nz - 1; ++k) suth = suth + U(1, k) * ds * ds * 0.5;
-  for (int k = 2; k <= Nz - 1; ++k) noth = noth + U(Ny, k) * ds * ds * 0.5;
-  for (int j = 2; j <= Ny - 1; ++j) back = back + U(j, 1) * ds * ds * 0.5;
-  for (int j = 2; j <= Ny - 1; ++j) frnt = frnt + U(j, Nz) * ds * ds * 0.5;
-  const double sb = U(1, 1) * ds * ds * 0.25, sf = U(1, Nz) * ds * ds * 0.25;
-  const double nb = U(Ny, 1) * ds * ds * 0.25, nf = U(Ny, Nz) * ds * ds * 0.25;
-  const double rl = 1.0 / c.prm.ref_l;
-  *m = (inner + suth + noth + frnt + back + sb + sf + nb + nf) * std::sqrt(1.0 / 2.0) /
-       c.prm.press_drop / (rl * rl);
-  return 0;
+void ctx_destroy(Ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  for (auto& g : c->graph)
+    if (g) hipGraphExecDestroy(g);
+  if (c->stream_ordered) {
+    for (void* p : c->allocs) hipFreeAsync(p, c->stream);
+  } else {
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+  }
+  for (auto& e : c->ev) hipEventDestroy(e);
+  if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+  delete c;
 }
 
-}  // namespace
+}  // namespace piks
 
 // ------------------------------------------------------------------ C ABI
 
@@ -988,41 +1000,55 @@ extern "C" {
 
 int piks_create(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes, void** ctx_out) {
   g_err.clear();
-  if (ctx_out) *ctx_out = nullptr;
+  if (!ctx_out) return fail(PIKS_E_ARG, "null argument");
+  *ctx_out = nullptr;
   Ctx* c = nullptr;
-  int rc = create(p, raw_image, image_bytes, &c);
-  if (rc == 0) *ctx_out = c;
-  return rc;
+  if (int rc = ctx_create_host(p, raw_image, image_bytes, &c)) return rc;
+  if (int rc = ctx_create_device(*c, nullptr)) {
+    ctx_destroy(c);
+    return rc;
+  }
+  *ctx_out = c;
+  return 0;
 }
 
 int piks_iterate(void* ctx, int32_t nsteps) {
   if (!ctx || nsteps < 0) return fail(PIKS_E_ARG, "bad context or step count");
-  return iterate(*static_cast<Ctx*>(ctx), nsteps);
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  if (c.use_graph) {
+    if (int rc = ctx_enqueue(c, nsteps)) return rc;
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    return 0;
+  }
+  return iterate(c, nsteps);
 }
 
 int piks_set_precision(void* ctx, int32_t precision) {
   if (!ctx) return fail(PIKS_E_ARG, "bad context");
   if (precision != PIKS_PREC_FP64 && precision != PIKS_PREC_FP32_REFINED)
     return fail(PIKS_E_ARG, "unknown precision mode " + std::to_string(precision));
-  return set_phase(*static_cast<Ctx*>(ctx), precision);
+  return ctx_set_phase(*static_cast<Ctx*>(ctx), precision);
+}
+
+static int mass_flow_sync(Ctx& c, double* m) {
+  std::vector<double> u((size_t)c.g.Ny * c.g.Nz);
+  if (int rc = ctx_enqueue_plane(c, u.data())) return rc;
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  *m = ctx_mass(c, u.data());
+  return 0;
 }
 
 int piks_mass_flow(void* ctx, double* m) {
   if (!ctx || !m) return fail(PIKS_E_ARG, "null argument");
-  return mass_flow(*static_cast<Ctx*>(ctx), m);
+  return mass_flow_sync(*static_cast<Ctx*>(ctx), m);
 }
 
 int piks_chk_converge(void* ctx, double out[3]) {
   if (!ctx || !out) return fail(PIKS_E_ARG, "null argument");
   Ctx& c = *static_cast<Ctx*>(ctx);
   double m = 0.0;
-  if (int rc = mass_flow(c, &m)) return rc;
-  // src/output.F90:151-155
-  const double error = std::fabs(1.0 - m / c.mass) / (double)c.prm.chk_converge_step;
-  c.mass = m;
-  out[0] = m;
-  out[1] = m * c.prm.kn * std::sqrt(4.0 / (std::atan(1.0) * 4.0));
-  out[2] = error;
+  if (int rc = mass_flow_sync(c, &m)) return rc;
+  ctx_check(c, m, out);  // src/output.F90:151-155
   return 0;
 }
 
@@ -1150,7 +1176,7 @@ int piks_last_timing(void* ctx, double ms[4]) {
 }
 
 int piks_destroy(void* ctx) {
-  free_ctx(static_cast<Ctx*>(ctx));
+  ctx_destroy(static_cast<Ctx*>(ctx));
   return 0;
 }
 

@@ -1,0 +1,415 @@
+// serve.hip -- continuous batching of independent solves on one GPU.
+//
+// The reference runs one solve per program (src/main.F90); a service that
+// answers many permeability requests needs them to share the device.  This
+// server keeps the reference's per-solve loop (iterate, chkConverge every
+// chkConvergeStep iterations, stop at error <= eps or maxStep) and runs many
+// such loops at once:
+//   * submit copies the request and queues it for host preprocessing
+//     (velocity grid, image labelling, sweep schedule) on a pool of threads;
+//   * the scheduler thread admits preprocessed requests in arrival order
+//     whenever a slot is free and their exact device footprint fits the
+//     memory budget, allocates them stream-ordered on the slot's stream,
+//     captures one iteration as a HIP graph, and enqueues iterations up to
+//     the next convergence check followed by an asynchronous copy of the
+//     check plane into pinned memory;
+//   * it polls the slots' events, evaluates each check on the host in the
+//     reference's order, and either enqueues the next chunk or retires the
+//     request, releasing its slot and memory to the next one in line.
+// Requests that arrive while others run are admitted as soon as resources
+// free up; their kernels overlap on the device through the slots' streams.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/piks3d.h"
+#include "context.h"
+
+namespace piks {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+  int64_t ticket = 0;
+  piks_params prm{};
+  std::vector<uint8_t> image;
+  int32_t max_step = 0;
+  double eps = 0.0, refine_switch = 1e-6;
+  int status = PIKS_REQ_QUEUED;
+  int error_code = 0;
+  std::string error_msg;
+  Ctx* ctx = nullptr;
+  size_t bytes = 0;
+  int slot = -1;
+  // solve state (src/main.F90:95-121)
+  int step = 0, steps32 = 0, chunk = 0, stalled = 0;
+  double best = std::numeric_limits<double>::infinity();
+  double mass = 1.0, perm = 0.0, error = 1.0;
+  bool converged = false, refined = false;
+  Clock::time_point t_submit, t_admit, t_done;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  double* plane = nullptr;  // pinned
+  size_t plane_cap = 0;
+  Request* req = nullptr;
+};
+
+struct Server {
+  piks_serve_params prm{};
+  size_t budget = 0, in_use = 0;
+  std::mutex mu;
+  std::condition_variable cv_prep, cv_sched, cv_done;
+  std::deque<Request*> prep_q, ready_q;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+  int64_t next_ticket = 1;
+  int pending = 0;  // submitted, not finished
+  bool stop = false;
+  std::vector<Slot> slots;
+  std::vector<std::thread> workers;
+  std::thread sched;
+  std::string sched_error;
+};
+
+void finish(Server& s, Request& r, int status) {
+  r.status = status;
+  r.t_done = Clock::now();
+  --s.pending;
+  s.cv_done.notify_all();
+}
+
+void prep_worker(Server* s) {
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_prep.wait(lk, [&] { return s->stop || !s->prep_q.empty(); });
+      if (s->stop) return;
+      r = s->prep_q.front();
+      s->prep_q.pop_front();
+    }
+    Ctx* c = nullptr;
+    const int rc = ctx_create_host(&r->prm, r->image.data(), (int64_t)r->image.size(), &c);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rc != 0) {
+      r->error_code = rc;
+      r->error_msg = piks_last_error();
+      finish(*s, *r, PIKS_REQ_FAILED);
+      continue;
+    }
+    c->use_graph = s->prm.use_graphs != 0;
+    r->ctx = c;
+    r->bytes = ctx_device_bytes(*c);
+    std::vector<uint8_t>().swap(r->image);
+    if (r->bytes > s->budget) {
+      r->error_code = PIKS_E_MEM;
+      r->error_msg = "request needs " + std::to_string(r->bytes >> 20) + " MiB, the server's budget is " +
+                     std::to_string(s->budget >> 20) + " MiB";
+      ctx_destroy(c);
+      r->ctx = nullptr;
+      finish(*s, *r, PIKS_REQ_FAILED);
+      continue;
+    }
+    s->ready_q.push_back(r);
+    s->cv_sched.notify_all();
+  }
+}
+
+// Iterations up to the next check (or maxStep), then the check plane.
+int enqueue_chunk(Slot& sl, Request& r) {
+  Ctx& c = *r.ctx;
+  const int chk = c.prm.chk_converge_step;
+  r.chunk = std::min(chk - r.step % chk, r.max_step - r.step);
+  if (int rc = ctx_enqueue(c, r.chunk)) return rc;
+  if (int rc = ctx_enqueue_plane(c, sl.plane)) return rc;
+  if (hipEventRecord(sl.done, sl.stream) != hipSuccess) return set_error(PIKS_E_HIP, "hipEventRecord failed");
+  return 0;
+}
+
+void fail_request(Server& s, Slot& sl, Request& r, int rc) {
+  r.error_code = rc;
+  r.error_msg = piks_last_error();
+  ctx_destroy(r.ctx);
+  r.ctx = nullptr;
+  s.in_use -= r.bytes;
+  sl.req = nullptr;
+  finish(s, r, PIKS_REQ_FAILED);
+}
+
+// A chunk finished: evaluate the check like the reference's loop.
+void on_chunk_done(Server& s, Slot& sl, Request& r) {
+  Ctx& c = *r.ctx;
+  r.step += r.chunk;
+  if (r.refined) r.steps32 += r.chunk;
+  const int chk = c.prm.chk_converge_step;
+  bool done = false;
+  if (r.step % chk == 0) {
+    double out[3];
+    ctx_check(c, ctx_mass(c, sl.plane), out);
+    r.mass = out[0];
+    r.perm = out[1];
+    r.error = out[2];
+    if (r.refined) {
+      // FP32 has gone as far as it can: below the switch level, or no
+      // improvement for three checks; finish in FP64
+      if (r.error < r.best) {
+        r.best = r.error;
+        r.stalled = 0;
+      } else {
+        ++r.stalled;
+      }
+      if (r.error <= std::max(r.refine_switch, r.eps) || r.stalled >= 3) {
+        if (int rc = ctx_set_phase(c, PIKS_PREC_FP64)) return fail_request(s, sl, r, rc);
+        r.refined = false;
+        r.error = 1.0;
+      }
+    } else if (r.error <= r.eps) {
+      r.converged = true;
+      done = true;
+    }
+  }
+  if (r.step >= r.max_step) done = true;
+  if (!done) {
+    if (int rc = enqueue_chunk(sl, r)) fail_request(s, sl, r, rc);
+    return;
+  }
+  r.mass = c.mass;
+  ctx_destroy(r.ctx);  // stream-ordered frees: no device-wide wait
+  r.ctx = nullptr;
+  s.in_use -= r.bytes;
+  sl.req = nullptr;
+  finish(s, r, PIKS_REQ_DONE);
+}
+
+void scheduler(Server* s) {
+  if (hipSetDevice(s->prm.device) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->sched_error = "hipSetDevice failed";
+    return;
+  }
+  for (;;) {
+    bool progressed = false;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->stop) return;
+    // admit in arrival order while a slot and the memory are free
+    while (!s->ready_q.empty()) {
+      Request* r = s->ready_q.front();
+      Slot* free_slot = nullptr;
+      for (auto& sl : s->slots)
+        if (!sl.req) {
+          free_slot = &sl;
+          break;
+        }
+      if (!free_slot || s->in_use + r->bytes > s->budget) break;
+      s->ready_q.pop_front();
+      Slot& sl = *free_slot;
+      const size_t need = (size_t)r->ctx->g.Ny * r->ctx->g.Nz;
+      if (need > sl.plane_cap) {
+        if (sl.plane) hipHostFree(sl.plane);
+        sl.plane = nullptr;
+        if (hipHostMalloc((void**)&sl.plane, need * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+          sl.plane_cap = 0;
+          r->error_code = PIKS_E_MEM;
+          r->error_msg = "pinned host allocation failed";
+          ctx_destroy(r->ctx);
+          r->ctx = nullptr;
+          finish(*s, *r, PIKS_REQ_FAILED);
+          continue;
+        }
+        sl.plane_cap = need;
+      }
+      r->slot = (int)(&sl - s->slots.data());
+      r->t_admit = Clock::now();
+      r->status = PIKS_REQ_RUNNING;
+      r->refined = r->prm.precision == PIKS_PREC_FP32_REFINED;
+      s->in_use += r->bytes;
+      sl.req = r;
+      int rc = ctx_create_device(*r->ctx, sl.stream);
+      if (rc == 0) rc = enqueue_chunk(sl, *r);
+      if (rc) fail_request(*s, sl, *r, rc);
+      progressed = true;
+    }
+    // retire or continue the running requests whose chunk has finished
+    for (auto& sl : s->slots) {
+      if (!sl.req) continue;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) {
+        set_error(PIKS_E_HIP, std::string("solve failed on the device: ") + hipGetErrorString(q));
+        fail_request(*s, sl, *sl.req, PIKS_E_HIP);
+        continue;
+      }
+      on_chunk_done(*s, sl, *sl.req);
+      progressed = true;
+    }
+    if (!progressed) {
+      bool running = false;
+      for (auto& sl : s->slots) running |= sl.req != nullptr;
+      if (running)
+        s->cv_sched.wait_for(lk, std::chrono::microseconds(50));
+      else
+        s->cv_sched.wait(lk, [&] { return s->stop || !s->ready_q.empty(); });
+    }
+  }
+}
+
+void fill_result(const Request& r, piks_serve_result* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->status = r.status;
+  out->error_code = r.error_code;
+  out->steps = r.step;
+  out->steps32 = r.steps32;
+  out->converged = r.converged ? 1 : 0;
+  out->permeability = r.perm;
+  out->mass = r.mass;
+  out->error = r.error;
+  out->device_bytes = (int64_t)r.bytes;
+  if (r.status == PIKS_REQ_DONE || r.status == PIKS_REQ_FAILED) {
+    const bool admitted = r.t_admit.time_since_epoch().count() != 0;
+    out->queue_ms = ms_between(r.t_submit, admitted ? r.t_admit : r.t_done);
+    out->run_ms = admitted ? ms_between(r.t_admit, r.t_done) : 0.0;
+    out->total_ms = ms_between(r.t_submit, r.t_done);
+  }
+}
+
+}  // namespace
+}  // namespace piks
+
+using namespace piks;
+
+extern "C" {
+
+int piks_serve_create(const piks_serve_params* sp, void** server_out) {
+  if (!server_out) return set_error(PIKS_E_ARG, "null argument");
+  *server_out = nullptr;
+  std::unique_ptr<Server> s(new Server());
+  if (sp) s->prm = *sp;
+  if (s->prm.max_slots <= 0) s->prm.max_slots = 8;
+  if (s->prm.prep_threads <= 0) s->prm.prep_threads = 4;
+  if (!sp) s->prm.use_graphs = 1;
+  if (s->prm.max_slots > 64) return set_error(PIKS_E_ARG, "at most 64 slots");
+  if (hipSetDevice(s->prm.device) != hipSuccess) return set_error(PIKS_E_HIP, "hipSetDevice failed");
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return set_error(PIKS_E_HIP, "hipMemGetInfo failed");
+  s->budget = s->prm.memory_budget > 0 ? (size_t)s->prm.memory_budget : free_b / 10 * 9;
+  if (s->budget > free_b) return set_error(PIKS_E_MEM, "memory budget exceeds the free device memory");
+  s->slots.resize((size_t)s->prm.max_slots);
+  for (auto& sl : s->slots) {
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+      return set_error(PIKS_E_HIP, "stream or event creation failed");
+  }
+  for (int q = 0; q < s->prm.prep_threads; ++q) s->workers.emplace_back(prep_worker, s.get());
+  s->sched = std::thread(scheduler, s.get());
+  *server_out = s.release();
+  return 0;
+}
+
+int piks_serve_submit(void* server, const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
+                      int32_t max_step, double eps, double refine_switch, int64_t* ticket_out) {
+  if (!server || !p || !raw_image || image_bytes < 0 || !ticket_out)
+    return set_error(PIKS_E_ARG, "null argument");
+  if (max_step < 1) return set_error(PIKS_E_ARG, "max_step must be >= 1");
+  if (p->chk_converge_step < 1) return set_error(PIKS_E_ARG, "chkConvergeStep must be >= 1");
+  Server& s = *static_cast<Server*>(server);
+  std::unique_ptr<Request> r(new Request());
+  r->prm = *p;
+  r->prm.device = s.prm.device;
+  r->image.assign(raw_image, raw_image + image_bytes);
+  r->max_step = max_step;
+  r->eps = eps;
+  r->refine_switch = refine_switch > 0 ? refine_switch : 1e-6;
+  r->t_submit = Clock::now();
+  std::lock_guard<std::mutex> lk(s.mu);
+  r->ticket = s.next_ticket++;
+  *ticket_out = r->ticket;
+  s.prep_q.push_back(r.get());
+  ++s.pending;
+  s.reqs[r->ticket] = std::move(r);
+  s.cv_prep.notify_one();
+  return 0;
+}
+
+int piks_serve_poll(void* server, int64_t ticket, piks_serve_result* out) {
+  if (!server || !out) return set_error(PIKS_E_ARG, "null argument");
+  Server& s = *static_cast<Server*>(server);
+  std::lock_guard<std::mutex> lk(s.mu);
+  auto it = s.reqs.find(ticket);
+  if (it == s.reqs.end()) return set_error(PIKS_E_ARG, "unknown ticket " + std::to_string(ticket));
+  fill_result(*it->second, out);
+  return 0;
+}
+
+int piks_serve_wait(void* server, int64_t ticket, piks_serve_result* out) {
+  if (!server || !out) return set_error(PIKS_E_ARG, "null argument");
+  Server& s = *static_cast<Server*>(server);
+  std::unique_lock<std::mutex> lk(s.mu);
+  auto it = s.reqs.find(ticket);
+  if (it == s.reqs.end()) return set_error(PIKS_E_ARG, "unknown ticket " + std::to_string(ticket));
+  Request& r = *it->second;
+  s.cv_done.wait(lk, [&] { return r.status == PIKS_REQ_DONE || r.status == PIKS_REQ_FAILED; });
+  fill_result(r, out);
+  return 0;
+}
+
+int piks_serve_drain(void* server) {
+  if (!server) return set_error(PIKS_E_ARG, "null argument");
+  Server& s = *static_cast<Server*>(server);
+  std::unique_lock<std::mutex> lk(s.mu);
+  s.cv_done.wait(lk, [&] { return s.pending == 0; });
+  return 0;
+}
+
+int piks_serve_destroy(void* server) {
+  if (!server) return 0;
+  Server* s = static_cast<Server*>(server);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_prep.notify_all();
+  s->cv_sched.notify_all();
+  for (auto& t : s->workers) t.join();
+  if (s->sched.joinable()) s->sched.join();
+  hipSetDevice(s->prm.device);
+  for (auto& sl : s->slots) {
+    if (sl.req && sl.req->ctx) {
+      hipStreamSynchronize(sl.stream);
+      ctx_destroy(sl.req->ctx);
+      sl.req->ctx = nullptr;
+    }
+  }
+  for (auto& kv : s->reqs)
+    if (kv.second->ctx) ctx_destroy(kv.second->ctx);
+  for (auto& sl : s->slots) {
+    if (sl.stream) {
+      hipStreamSynchronize(sl.stream);
+      hipStreamDestroy(sl.stream);
+    }
+    if (sl.done) hipEventDestroy(sl.done);
+    if (sl.plane) hipHostFree(sl.plane);
+  }
+  delete s;
+  return 0;
+}
+
+}  // extern "C"
