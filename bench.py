@@ -90,6 +90,75 @@ def traffic_from_profiles(workload_name):
     return None
 
 
+SERVE_SIDES = (32, 48, 64, 96, 128)
+
+
+def serve_workload(nreq, seed, max_step, eps):
+    """The serving stream: nreq seeded sphere packs, side uniform over
+    SERVE_SIDES, porosity U[0.3, 0.5], Kn log-uniform over [0.01, 10], velocity
+    grid 6^3 for Kn < 0.1, 8^3 for Kn < 1, else 12^3 (half-range), 4 fluid
+    layers, unit pressure drop."""
+    import numpy as np
+    from piks3d_amd import images, preprocess
+    from piks3d_amd.params import Params
+    rng = np.random.Generator(np.random.PCG64(seed))
+    reqs = []
+    for q in range(nreq):
+        side = int(rng.choice(SERVE_SIDES))
+        phi = float(rng.uniform(0.3, 0.5))
+        kn = float(10.0 ** rng.uniform(-2.0, 1.0))
+        nc = 3 if kn < 0.1 else (4 if kn < 1.0 else 6)
+        img = images.sphere_pack(side, phi, 4.0, 10.0, seed=seed * 100003 + q)
+        p = Params(nx=side, ny=side, nz=side, fluid_layer=4, wall_ext_order=2, nc_fundamental=nc,
+                   half_range=True, exact_quadrature=(nc == 3), chk_converge_step=10,
+                   max_step=max_step, eps=eps, kn=kn, press_drop=1.0, accom=1.0)
+        _, info = preprocess(p, img)
+        reqs.append((p, img, info.nfluid * 8 * nc ** 3))
+    return reqs
+
+
+def serve_bench(args):
+    """Poisson arrivals into the request server; one JSON line."""
+    import numpy as np
+    from piks3d_amd import REQ_DONE, Server
+    log("generating %d requests" % args.requests)
+    reqs = serve_workload(args.requests, args.seed, args.max_step, args.eps)
+    rng = np.random.Generator(np.random.PCG64(args.seed + 1))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, len(reqs)))
+    with Server(max_slots=args.slots, prep_threads=args.prep_threads) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for (p, img, _), at in zip(reqs, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(p, img, precision=args.precision))
+        log("all %d submitted after %.1f s" % (len(tickets), time.perf_counter() - t0))
+        res = []
+        for q, t in enumerate(tickets):
+            res.append(srv.wait(t))
+            if (q + 1) % 32 == 0:
+                log("%d done, %.1f s" % (q + 1, time.perf_counter() - t0))
+        wall = time.perf_counter() - t0
+    ok = [r for r in res if r["status"] == REQ_DONE]
+    lat = np.array([r["total_ms"] for r in ok])
+    updates = sum(u * r["steps"] for (_, _, u), r in zip(reqs, res) if r["status"] == REQ_DONE)
+    out = {
+        "metric": "batched solves/s + p99 latency (serving stream)", "value": len(ok) / wall,
+        "unit": "solves/s", "higher_is_better": True, "n_gpus": 1,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "mean_run_ms": float(np.mean([r["run_ms"] for r in ok])),
+        "updates_per_s": updates / wall, "wall_s": wall, "failed": len(res) - len(ok),
+        "converged": int(sum(r["converged"] for r in ok)),
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": "serving_stream_%d" % args.requests, "sides": list(SERVE_SIDES),
+                   "porosity": [0.3, 0.5], "kn": [0.01, 10], "arrival_rate_per_s": args.rate,
+                   "slots": args.slots, "max_step": args.max_step, "eps": args.eps,
+                   "precision": args.precision, "seed": args.seed},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +170,15 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--serve", action="store_true",
+                    help="run the serving stream instead (Poisson arrivals into the request server)")
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=8.0, help="serving: mean arrivals per second")
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--prep-threads", type=int, default=8)
+    ap.add_argument("--max-step", type=int, default=400)
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -115,6 +193,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copied %d bytes three times" % n)
+        return
+    if args.serve:
+        serve_bench(args)
         return
 
     try:
