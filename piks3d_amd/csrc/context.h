@@ -40,6 +40,14 @@ struct alignas(32) TaskRec {
   int32_t w[8];
 };
 
+// Storage window of one (hyperplane, octant) as the sweep addresses it: the
+// window's first element in f and its length, both in elements of f.
+struct alignas(16) SweepWin {
+  int64_t off;
+  uint32_t len;
+  uint32_t pad;
+};
+
 // Device state that carries from one iteration to the next, in one precision.
 template <typename T>
 struct State {
@@ -85,7 +93,10 @@ struct Ctx {
   bool xcd_remap = true;       // contiguous task ranges per XCD
   int sweep_block = 64;        // threads per sweep workgroup (one wave: a task's
                                // workgroup ends with its wave, measured fastest)
+  bool sweep_vload = false;    // task records and moments through the vector memory path
   int32_t* hp_task = nullptr;
+  SweepWin* hp_win = nullptr;  // storage window per (hyperplane, octant)
+  void* view_args[2] = {nullptr, nullptr};  // per precision, device copies for the sweep
   int32_t* pos[4] = {nullptr, nullptr, nullptr, nullptr};
   // one iteration captured as a HIP graph, per precision (used by ctx_enqueue
   // when use_graph is set)

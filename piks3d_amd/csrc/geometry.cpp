@@ -2,6 +2,7 @@
 #include "geometry.h"
 
 #include <algorithm>
+#include <climits>
 #include <array>
 #include <cstring>
 #include <unordered_map>
@@ -417,6 +418,25 @@ void build_sweep_schedule(Geometry& g) {
             t[6] = pos[(size_t)(c - 2 * dz)];
             t[7] = (int32_t)((uint32_t)c | flags << 28);
           }
+    }
+  }
+  // storage windows
+  g.hp_lo.assign((size_t)nb, 0);
+  g.hp_len.assign((size_t)nb, 0);
+  for (int64_t b = 0; b < nb; ++b) {
+    int32_t lo = INT32_MAX, hi = -1;
+    for (int64_t t = count[(size_t)b]; t < count[(size_t)b + 1]; ++t) {
+      const int32_t* r = &g.hp_task[(size_t)(t * W)];
+      for (int q = 0; q < 7; ++q) {
+        const int32_t v = q == 0 ? (int32_t)((uint32_t)r[0] & 0x0fffffffu) : r[q];
+        if (v == (int32_t)ns) continue;
+        lo = std::min(lo, v);
+        hi = std::max(hi, v);
+      }
+    }
+    if (hi >= lo) {
+      g.hp_lo[(size_t)b] = lo;
+      g.hp_len[(size_t)b] = hi - lo + 1;
     }
   }
 }

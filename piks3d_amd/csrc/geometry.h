@@ -80,6 +80,9 @@ struct Geometry {
   int nhp = 0;
   std::vector<int32_t> hp_task;
   std::vector<int64_t> hp_off;
+  // Storage window of each (hyperplane, octant): every storage index its
+  // tasks touch (the zero slot aside) lies in [hp_lo, hp_lo + hp_len).
+  std::vector<int32_t> hp_lo, hp_len;  // nhp * 8
 
   int64_t cell(int i, int j, int k) const {
     return (int64_t)(i + 1) + (int64_t)(j + 1) * Nxt + (int64_t)(k + 1) * Nxyt;

@@ -92,24 +92,36 @@ __device__ __forceinline__ size_t fat(const Dims& d, int o, int64_t cell) {
 // ------------------------------------------------------------------ sweep
 
 template <typename T>
+struct ViewArgs;
+
+template <typename T>
 struct SweepArgs {
   T* f;
   const Mom* mom;
   double* part;            // [cell][octant][4] partial moments; null = skip
-  const int32_t* wid;
-  const uint64_t* views;
-  const int32_t* wall_corner;
-  const T* fw;             // [corner][octant][velocity]
-  const uint8_t* lab;
+  const TaskRec* tasks;    // this hyperplane's task records
+  const SweepWin* win;     // this hyperplane's storage window per octant
   const double* cx;
   const double* cy;
   const double* cz;
   const double* w;
   double mu, c0, c1, c2, d1;  // 1.5/ds, 2/ds, -0.5/ds, 1/ds
+  const ViewArgs<T>* view; // what the rare switched-wall reads need
+  int32_t ntask;
+  int32_t Nc8;
+  int32_t xcd_remap;
+};
+
+// Everything a read of a switched wall slot needs (device memory).
+template <typename T>
+struct ViewArgs {
+  const T* f;
+  const T* fw;             // [corner][octant][velocity]
+  const uint8_t* lab;
+  const int32_t* wid;
+  const uint64_t* views;
+  const int32_t* wall_corner;
   Dims d;
-  const TaskRec* tasks;    // this hyperplane's task records
-  int64_t ntask;
-  bool xcd_remap;
 };
 
 
@@ -117,7 +129,7 @@ struct SweepArgs {
 // o, when the reference would have switched that wall slot by then; null if
 // the plain f slot applies.
 template <typename T>
-__device__ __forceinline__ const T* switched_source(const SweepArgs<T>& a, int64_t n, int o,
+__device__ __forceinline__ const T* switched_source(const ViewArgs<T>& a, int64_t n, int o,
                                                     int axis) {
   if (a.lab[n] < 20) return nullptr;
   const int w = a.wid[n];
@@ -129,14 +141,15 @@ __device__ __forceinline__ const T* switched_source(const SweepArgs<T>& a, int64
 
 // Sum of four per-lane values over the wave in 7 exchanges: lane 0 ends with
 // the sum of a, lane 16 of b, lane 32 of c, lane 48 of d.
-__device__ __forceinline__ double wave_sum4(double a, double b, double c, double d, int lane) {
+template <typename C>
+__device__ __forceinline__ C wave_sum4(C a, C b, C c, C d, int lane) {
   const bool up = lane & 32;
-  const double r0 = __shfl_xor(up ? a : c, 32, kWave);
-  const double r1 = __shfl_xor(up ? b : d, 32, kWave);
-  const double x = (up ? c : a) + r0;
-  const double y = (up ? d : b) + r1;
+  const C r0 = __shfl_xor(up ? a : c, 32, kWave);
+  const C r1 = __shfl_xor(up ? b : d, 32, kWave);
+  const C x = (up ? c : a) + r0;
+  const C y = (up ? d : b) + r1;
   const bool b4 = lane & 16;
-  double z = (b4 ? y : x) + __shfl_xor(b4 ? x : y, 16, kWave);
+  C z = (b4 ? y : x) + __shfl_xor(b4 ? x : y, 16, kWave);
   z += __shfl_xor(z, 8, kWave);
   z += __shfl_xor(z, 4, kWave);
   z += __shfl_xor(z, 2, kWave);
@@ -149,74 +162,155 @@ __device__ __forceinline__ double wave_sum4(double a, double b, double c, double
 // the launch's tasks: consecutive tasks of one octant share upwind
 // neighbours, which then stay in that XCD's L2.  Launch grids are multiples
 // of 8 blocks.
-__device__ __forceinline__ int64_t xcd_block(bool remap) {
+__device__ __forceinline__ uint32_t xcd_block(bool remap) {
   if (!remap) return blockIdx.x;
-  const int per = gridDim.x / 8;
-  return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  const uint32_t per = gridDim.x / 8;
+  return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+// Raw (untyped) buffer resources, word 3 for the gfx9 family: loads outside
+// the resource's byte range return 0 and stores outside it are dropped.
+constexpr int kBufferWord3 = 0x00020000;
+
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+
+// Values seen by a cell whose y or z upwind neighbours are switched wall
+// slots (out of line: few tasks take this path).
+template <typename T>
+struct ViewFix {
+  T v[4];                  // y1 y2 z1 z2
+  uint32_t mask;           // which of them replace the plain slots
+};
+
+template <typename T>
+__device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell, int o, int ll) {
+  const ViewArgs<T>& a = *va;
+  const int64_t dy = (int64_t)osy(o) * a.d.Nxt, dz = (int64_t)osz(o) * a.d.Nxyt;
+  const int64_t nbr[4] = {(int64_t)cell - dy, (int64_t)cell - 2 * dy, (int64_t)cell - dz, (int64_t)cell - 2 * dz};
+  ViewFix<T> r{};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T* src = switched_source(a, nbr[q], o, q < 2 ? 1 : 2);
+    if (src) {
+      r.v[q] = src[ll];
+      r.mask |= 1u << q;
+    }
+  }
+  return r;
 }
 
 // One hyperplane of the eight octant sweeps.  Every task (cell, octant) has
 // its upwind neighbours on the two previous hyperplanes (earlier launches), so
 // no wave waits on another.  A wave updates one task, all its velocities (64
-// per pass); the task record holds the storage indices of the cell and of its
-// six upwind neighbours, so all f loads are issued at once.
+// per pass).  The task record holds the element offsets of the cell and of its
+// six upwind neighbours inside the octant's storage window for this
+// hyperplane, so every f access is a buffer access with a scalar offset and
+// the lane's velocity as vector offset; the zero slot is encoded as the
+// window's end, where the buffer returns 0.
 // Each update is src/solver.F90:206-214 (and analogues) with the same
 // operations in the same order (contraction off): signs enter as exact
 // negations, and a first-order axis contributes (c*0)*f like the reference.
-template <typename T, int kThreads>
+template <typename T, int kThreads, bool kVecLoad>
 __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
+  // the update runs in FP64 in both phases: FP32 arithmetic leaves a larger
+  // error along the iteration's slowest mode than FP32 storage alone
+  using C = double;
   // the wave index is uniform over the wave: keep it (and everything derived
   // from it: the record, cell, octant, the uniform parts of addresses) in SGPRs
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int64_t t = xcd_block(a.xcd_remap) * (kThreads / kWave) + wave;
-  if (t >= a.ntask) return;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint32_t t = xcd_block(a.xcd_remap) * (kThreads / kWave) + wave;
+  if (t >= (uint32_t)a.ntask) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const TaskRec r = ld_const(a.tasks + t);
+  // the record and the moments come either through the scalar cache or
+  // through the vector memory path (lanes 0-7 load the record's words, lanes
+  // 0-3 the moments, broadcast with readlane)
+  TaskRec r;
+  if (kVecLoad) {
+    const int32_t rw = lane < 8 ? reinterpret_cast<const int32_t*>(a.tasks + t)[lane] : 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.w[q] = __builtin_amdgcn_readlane(rw, q);
+  } else {
+    r = ld_const(a.tasks + t);
+  }
   const int o = (int)((uint32_t)r.w[0] >> 28);
-  const int64_t self = r.w[0] & 0x0fffffff;
-  const int64_t cell = r.w[7] & 0x0fffffff;
+  const uint32_t self = ((uint32_t)r.w[0] & 0x0fffffffu) * (uint32_t)sizeof(T);
+  const uint32_t cell = (uint32_t)r.w[7] & 0x0fffffffu;
   const uint32_t flg = (uint32_t)r.w[7] >> 28;
-  const Mom m = ld_const(a.mom + cell);
+  Mom m;
+  if (!kVecLoad) {
+    m = ld_const(a.mom + cell);
+  } else {
+    const double mvl = lane < 4 ? reinterpret_cast<const double*>(a.mom + cell)[lane] : 0.0;
+    const int64_t bits = __builtin_bit_cast(int64_t, mvl);
+    const int lo = (int)(bits & 0xffffffff), hi = (int)(bits >> 32);
+    auto pick = [&](int q) {
+      const uint32_t l32 = (uint32_t)__builtin_amdgcn_readlane(lo, q);
+      const uint32_t h32 = (uint32_t)__builtin_amdgcn_readlane(hi, q);
+      return __builtin_bit_cast(double, ((uint64_t)h32 << 32) | l32);
+    };
+    m.r = pick(0); m.x = pick(1); m.y = pick(2); m.z = pick(3);
+  }
+  const SweepWin win = ld_const(a.win + o);
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(a.f + win.off, 0, (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
   const int sx = osx(o), sy = osy(o), sz = osz(o);
-  const int Nc8 = a.d.Nc8;
-  T* fo = a.f + (size_t)o * a.d.ostride;
+  const int Nc8 = a.Nc8;
+  const C mx = (C)m.x, my = (C)m.y, mz = (C)m.z, mr = (C)m.r, mu = (C)a.mu;
   for (int base = 0; base < Nc8; base += kWave) {
     const int l = base + lane;
     const bool lact = l < Nc8;
     const int ll = lact ? l : 0;
-    const T fp = fo[self * Nc8 + ll];
+    const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
+    const T fp = buf_load<T>(rs, voff, self);
     T nb[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) nb[q] = fo[(int64_t)r.w[1 + q] * Nc8 + ll];
+    for (int q = 0; q < 6; ++q) nb[q] = buf_load<T>(rs, voff, (uint32_t)r.w[1 + q] * (uint32_t)sizeof(T));
     if (flg & 8u) {  // reads a wall slot the reference switched
-      const int64_t dy = (int64_t)sy * a.d.Nxt, dz = (int64_t)sz * a.d.Nxyt;
-      const T* q;
-      if ((q = switched_source(a, cell - dy, o, 1))) nb[2] = q[ll];
-      if ((q = switched_source(a, cell - 2 * dy, o, 1))) nb[3] = q[ll];
-      if ((q = switched_source(a, cell - dz, o, 2))) nb[4] = q[ll];
-      if ((q = switched_source(a, cell - 2 * dz, o, 2))) nb[5] = q[ll];
+      const ViewFix<T> vf = view_fix(a.view, cell, o, ll);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (vf.mask & (1u << q)) nb[2 + q] = vf.v[q];
     }
-    const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
+    const C cxl = (C)a.cx[ll], cyl = (C)a.cy[ll], czl = (C)a.cz[ll], wl = (C)a.w[ll];
     const bool fx = flg & 1u, fy = flg & 2u, fz = flg & 4u;
-    double tt = sx > 0 ? cxl * m.x : -(cxl * m.x);
-    tt = sy > 0 ? tt + cyl * m.y : tt - cyl * m.y;
-    tt = sz > 0 ? tt + czl * m.z : tt - czl * m.z;
-    const double feq = wl * (m.r + 1.0 * tt);
-    double num = a.mu * (feq - 0.5 * (double)fp);
-    num = num + cxl * (fx ? a.d1 : a.c1) * (double)nb[0];
-    num = num + cxl * (fx ? 0.0 : a.c2) * (double)nb[1];
-    num = num + cyl * (fy ? a.d1 : a.c1) * (double)nb[2];
-    num = num + cyl * (fy ? 0.0 : a.c2) * (double)nb[3];
-    num = num + czl * (fz ? a.d1 : a.c1) * (double)nb[4];
-    num = num + czl * (fz ? 0.0 : a.c2) * (double)nb[5];
-    double den = 0.5 * a.mu + cxl * (fx ? a.d1 : a.c0);
-    den = den + cyl * (fy ? a.d1 : a.c0);
-    den = den + czl * (fz ? a.d1 : a.c0);
-    const T fnew = (T)(num / den);
-    if (lact) fo[self * Nc8 + ll] = fnew;
+    const C d1 = (C)a.d1, c0 = (C)a.c0, c1 = (C)a.c1, c2 = (C)a.c2;
+    C tt = sx > 0 ? cxl * mx : -(cxl * mx);
+    tt = sy > 0 ? tt + cyl * my : tt - cyl * my;
+    tt = sz > 0 ? tt + czl * mz : tt - czl * mz;
+    const C feq = wl * (mr + C(1) * tt);
+    C num = mu * (feq - C(0.5) * fp);
+    num = num + cxl * (fx ? d1 : c1) * nb[0];
+    num = num + cxl * (fx ? C(0) : c2) * nb[1];
+    num = num + cyl * (fy ? d1 : c1) * nb[2];
+    num = num + cyl * (fy ? C(0) : c2) * nb[3];
+    num = num + czl * (fz ? d1 : c1) * nb[4];
+    num = num + czl * (fz ? C(0) : c2) * nb[5];
+    C den = C(0.5) * mu + cxl * (fx ? d1 : c0);
+    den = den + cyl * (fy ? d1 : c0);
+    den = den + czl * (fz ? d1 : c0);
+    const C fnew = num / den;
+    if (lact) buf_store((T)fnew, rs, voff, self);
     if (a.part) {
-      const double fv = lact ? (double)fnew : 0.0;
-      const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv, lane);
+      // moments are summed in FP64 in both phases: the mass balance they carry
+      // into the next iteration is what the slowest mode of the iteration feels
+      const double fv = lact ? (double)(T)fnew : 0.0;
+      const double z = wave_sum4<double>(fv, cxl * fv, cyl * fv, czl * fv, lane);
       if ((lane & 15) == 0) {
         const int q = lane >> 4;
         const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
@@ -657,6 +751,24 @@ WallArgs<T> wall_args(Ctx& c, State<T>& st, const int32_t* list, int n) {
   return a;
 }
 
+// The sweep kernel for a workgroup size.
+template <typename T>
+using SweepKernel = void (*)(SweepArgs<T>);
+
+template <typename T, bool V>
+SweepKernel<T> sweep_kernel_v(int block) {
+  switch (block) {
+    case 128: return k_sweep<T, 128, V>;
+    case 256: return k_sweep<T, 256, V>;
+    default: return k_sweep<T, 64, V>;
+  }
+}
+
+template <typename T>
+SweepKernel<T> sweep_kernel(int block, bool vec_load) {
+  return vec_load ? sweep_kernel_v<T, true>(block) : sweep_kernel_v<T, false>(block);
+}
+
 // Enqueue nsteps iterations on the context's stream.  With `timed`, four
 // events per step (at most kMaxTimedSteps) bracket the sweep, boundary and
 // moment phases; the caller synchronises and reads them.
@@ -665,15 +777,15 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
   const Geometry& g = c.g;
   const Dims d = c.d;
   SweepArgs<T> sa{};
-  sa.f = st.f; sa.lab = c.lab;
-  sa.xcd_remap = c.xcd_remap;
+  sa.f = st.f;
   sa.mom = c.mom;
   sa.part = c.exact ? nullptr : c.part;
-  sa.wid = c.wid; sa.views = c.views; sa.wall_corner = c.wall_corner; sa.fw = st.fw;
   sa.cx = c.dcx; sa.cy = c.dcy; sa.cz = c.dcz; sa.w = c.dw;
   sa.mu = c.mu;
   sa.c0 = 1.5 / g.ds; sa.c1 = 2.0 / g.ds; sa.c2 = -0.5 / g.ds; sa.d1 = 1.0 / g.ds;
-  sa.d = d;
+  sa.view = static_cast<const ViewArgs<T>*>(c.view_args[sizeof(T) == sizeof(double) ? 0 : 1]);
+  sa.Nc8 = d.Nc8;
+  sa.xcd_remap = c.xcd_remap;
   const int wpb = 4;  // waves per block for one-wave-per-item kernels
   const int timed = timed_run ? std::min(nsteps, kMaxTimedSteps) : 0;
   while ((int)c.ev.size() < 4 * timed) {
@@ -685,17 +797,14 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
     const bool tm = step < timed;
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 0], c.stream));
     for (int h = 0; h < g.nhp; ++h) {
-      sa.ntask = g.hp_off[h + 1] - g.hp_off[h];
+      sa.ntask = (int32_t)(g.hp_off[h + 1] - g.hp_off[h]);
       if (sa.ntask == 0) continue;
       sa.tasks = reinterpret_cast<const TaskRec*>(c.hp_task) + g.hp_off[h];
+      sa.win = c.hp_win + (size_t)h * 8;
       const int64_t wpg = c.sweep_block / kWave;
       const int64_t blocks = ((sa.ntask + wpg - 1) / wpg + 7) / 8 * 8;
-      auto kern = c.sweep_block == 64    ? k_sweep<T, 64>
-                  : c.sweep_block == 128 ? k_sweep<T, 128>
-                  : c.sweep_block == 512 ? k_sweep<T, 512>
-                  : c.sweep_block == 1024 ? k_sweep<T, 1024>
-                                          : k_sweep<T, 256>;
-      hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(c.sweep_block), 0, c.stream, sa);
+      hipLaunchKernelGGL(sweep_kernel<T>(c.sweep_block, c.sweep_vload), dim3((unsigned)blocks),
+                         dim3(c.sweep_block), 0, c.stream, sa);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
@@ -803,12 +912,31 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   if (const char* b = std::getenv("PIKS_XCD_REMAP")) c->xcd_remap = std::atoi(b) != 0;
   if (const char* b = std::getenv("PIKS_SWEEP_BLOCK")) {
     const int v = std::atoi(b);
-    if (v != 64 && v != 128 && v != 256 && v != 512 && v != 1024)
-      return fail(PIKS_E_ARG, "PIKS_SWEEP_BLOCK must be 64, 128, 256, 512 or 1024");
+    if (v != 64 && v != 128 && v != 256) return fail(PIKS_E_ARG, "PIKS_SWEEP_BLOCK must be 64, 128 or 256");
     c->sweep_block = v;
   }
+
   if (const char* b = std::getenv("PIKS_GRAPH")) c->use_graph = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_SWEEP_VLOAD")) c->sweep_vload = std::atoi(b) != 0;
   c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
+  // the sweep addresses one hyperplane's storage window with 32-bit offsets
+  int64_t maxwin = 0;
+  for (int32_t v : g.hp_len) maxwin = std::max<int64_t>(maxwin, v);
+  if (maxwin * c->d.Nc8 * (int64_t)sizeof(double) >= ((int64_t)1 << 31))
+    return fail(PIKS_E_ARG, "a hyperplane's storage window exceeds 2 GiB (image or velocity grid too large)");
+  // task records as the sweep reads them: element offsets inside the
+  // (hyperplane, octant) window, the zero slot as the window's end
+  Geometry& gm = c->g;
+  const int64_t nc8 = c->d.Nc8;
+  for (int h = 0; h < gm.nhp; ++h)
+    for (int64_t t = gm.hp_off[(size_t)h]; t < gm.hp_off[(size_t)h + 1]; ++t) {
+      int32_t* w = &gm.hp_task[(size_t)(t * Geometry::kTaskWords)];
+      const int o = (int)((uint32_t)w[0] >> 28);
+      const int64_t lo = gm.hp_lo[(size_t)h * 8 + o], len = gm.hp_len[(size_t)h * 8 + o];
+      auto rel = [&](int64_t p) { return (p == gm.nstore ? len : p - lo) * nc8; };
+      w[0] = (int32_t)((uint32_t)rel((uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
+      for (int q = 1; q <= 6; ++q) w[q] = (int32_t)rel(w[q]);
+    }
   *out = c.release();
   return 0;
 }
@@ -822,7 +950,7 @@ size_t ctx_device_bytes(const Ctx& c) {
   const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.uncoupled.size() * 4 +
                        g.coupled.size() * 4 * 7 + g.snap_cell.size() * 4 + g.commit_walls.size() * 4;
   const size_t fluid = (size_t)(g.fluid_interior.size() + g.fluid_boundary.size()) * 8;
-  return state64 + state32 + per_cell + walls + fluid + g.hp_task.size() * 4 +
+  return state64 + state32 + per_cell + walls + fluid + g.hp_task.size() * 4 + g.hp_lo.size() * 8 +
          (size_t)g.Ny * g.Nz * 8 + 4 * c.cx.size() * 8 + 64 * 256;
 }
 
@@ -875,9 +1003,29 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
   all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
   if ((rc = upload(c, &c.fluid_all, all))) return rc;
   if ((rc = upload(c, &c.hp_task, g.hp_task))) return rc;
+  std::vector<SweepWin> win(g.hp_lo.size());
+  for (size_t q = 0; q < win.size(); ++q) {
+    const int o = (int)(q % 8);
+    win[q].off = (int64_t)o * c.d.ostride + (int64_t)g.hp_lo[q] * c.d.Nc8;
+    win[q].len = (uint32_t)((int64_t)g.hp_len[q] * c.d.Nc8);
+    win[q].pad = 0;
+  }
+  if ((rc = upload(c, &c.hp_win, win))) return rc;
   for (int F = 0; F < 4; ++F) {
     if ((rc = upload(c, &c.pos[F], g.pos[F]))) return rc;
     c.d.pos[F] = c.pos[F];
+  }
+  {
+    std::vector<ViewArgs<double>> v64(1, ViewArgs<double>{c.s64.f, c.s64.fw, c.lab, c.wid, c.views, c.wall_corner, c.d});
+    ViewArgs<double>* p64 = nullptr;
+    if ((rc = upload(c, &p64, v64))) return rc;
+    c.view_args[0] = p64;
+    if (refined) {
+      std::vector<ViewArgs<float>> v32(1, ViewArgs<float>{c.s32.f, c.s32.fw, c.lab, c.wid, c.views, c.wall_corner, c.d});
+      ViewArgs<float>* p32 = nullptr;
+      if ((rc = upload(c, &p32, v32))) return rc;
+      c.view_args[1] = p32;
+    }
   }
   if (c.own_stream) HIP_TRY(hipStreamSynchronize(c.stream));
   return 0;
