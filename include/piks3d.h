@@ -137,6 +137,21 @@ int piks_get_info(void* ctx, piks_info* info);
 int piks_preprocess(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
                     int32_t* labels, piks_info* info);
 
+/* Several right-hand sides on one geometry: nrhs solves of the same image,
+ * velocity grid and parameters that differ only in Kn (kn[0..nrhs-1]; p->kn is
+ * ignored), advanced together by piks_iterate with one sweep launch per
+ * hyperplane for all of them (each task's geometry record is read once for
+ * all right-hand sides).  No counterpart in the reference, which solves one Kn
+ * per run; each right-hand side's iterates are those of piks_create with its
+ * Kn.  piks_chk_converge_multi fills out[3*r + 0..2] = mass, permeability,
+ * error of right-hand side r (src/output.F90:151-155 for each); the other
+ * calls (piks_chk_converge, piks_get_macro, piks_get_f, ...) see right-hand
+ * side 0. */
+int piks_create_multi(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
+                      int32_t nrhs, const double* kn, void** ctx_out);
+int piks_chk_converge_multi(void* ctx, double* out);
+int piks_get_macro_rhs(void* ctx, int32_t rhs, double* rho, double* ux, double* uy, double* uz);
+
 /* Device time of the last piks_iterate call, summed over its steps (the first
  * 4096) and split by phase (milliseconds): [0] sweeps, [1] switch commit +
  * exchange + wall BC + inlet/outlet + symmetry, [2] moments, [3] first step's

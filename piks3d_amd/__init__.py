@@ -99,6 +99,11 @@ _API = {
     "piks_last_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "piks_last_error": (ctypes.c_char_p, []),
+    "piks_create_multi": (ctypes.c_int, [ctypes.POINTER(PiksParams), ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "piks_chk_converge_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "piks_get_macro_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 4),
     "piks_serve_create": (ctypes.c_int, [ctypes.POINTER(ServeParams), ctypes.POINTER(ctypes.c_void_p)]),
     "piks_serve_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PiksParams), ctypes.c_void_p,
                                          ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
@@ -365,3 +370,81 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class MultiSolver(Solver):
+    """Several Knudsen numbers on one geometry (piks_create_multi): one
+    context, one sweep per hyperplane for all right-hand sides."""
+
+    def __init__(self, params: Params, image, kn_values, precision=None, exact_order=False, device=None):
+        self.params = params
+        self.kn_values = [float(k) for k in kn_values]
+        raw = np.ascontiguousarray(np.asarray(image, dtype=np.uint8).ravel())
+        self._cp = to_cparams(params, precision, exact_order, device)
+        kn = (ctypes.c_double * len(self.kn_values))(*self.kn_values)
+        h = ctypes.c_void_p()
+        _check(lib().piks_create_multi(ctypes.byref(self._cp), raw.ctypes.data, raw.size,
+                                       len(self.kn_values), kn, ctypes.byref(h)))
+        self._h = h
+        self.info = self.get_info()
+        self.mass = 1.0
+        self.history = []
+
+    @property
+    def nrhs(self):
+        return len(self.kn_values)
+
+    def chk_converge_all(self):
+        """(mass, permeability, error) arrays over the right-hand sides."""
+        out = (ctypes.c_double * (3 * self.nrhs))()
+        _check(lib().piks_chk_converge_multi(self._h, out))
+        a = np.array(out[:]).reshape(self.nrhs, 3)
+        return a[:, 0], a[:, 1], a[:, 2]
+
+    def macro_rhs(self, r):
+        n = self.info.ntotal
+        arrs = [np.empty(n) for _ in range(4)]
+        _check(lib().piks_get_macro_rhs(self._h, int(r), *[a.ctypes.data for a in arrs]))
+        return tuple(arrs)
+
+    def solve(self, max_step=None, chk_step=None, eps=None, callback=None, refine_switch=None):
+        """The reference's loop for every right-hand side at once: stops when
+        each has error <= eps (or at max_step).  In an FP32-refined context the
+        switch to FP64 happens when every right-hand side has reached the
+        switch criterion."""
+        p = self.params
+        max_step = p.max_step if max_step is None else max_step
+        chk = p.chk_converge_step if chk_step is None else chk_step
+        eps = p.eps if eps is None else eps
+        tau = p.refine_switch if refine_switch is None else refine_switch
+        refined = self.info.precision == PIKS_PREC_FP32_REFINED
+        steps32 = 0
+        best = np.full(self.nrhs, np.inf)
+        stalled = np.zeros(self.nrhs, dtype=int)
+        error = np.ones(self.nrhs)
+        perm = np.zeros(self.nrhs)
+        step = 0
+        while step < max_step:
+            n = min(chk - step % chk, max_step - step)
+            self.iterate(n)
+            step += n
+            if refined:
+                steps32 += n
+            if step % chk:
+                continue
+            m, perm, error = self.chk_converge_all()
+            self.history.append((step, m.copy(), perm.copy(), error.copy()))
+            if callback:
+                callback(step, m, perm, error)
+            if refined:
+                improved = error < best
+                best = np.where(improved, error, best)
+                stalled = np.where(improved, 0, stalled + 1)
+                if np.all((error <= max(tau, eps)) | (stalled >= 3)):
+                    self.set_precision("fp64")
+                    refined = False
+                continue
+            if np.all(error <= eps):
+                break
+        return dict(steps=step, steps32=steps32, kn=list(self.kn_values), permeability=perm, error=error,
+                    history=list(self.history))

@@ -27,6 +27,9 @@ struct Dims {
   // elements per octant
   const int32_t* pos[4];
   int64_t ostride;
+  // right-hand sides (Knudsen numbers) sharing the geometry: kernels take the
+  // one of blockIdx.y; element strides between them
+  int64_t rs_f, rs_fw, rs_buf, rs_snap;
 };
 
 // Moments of one cell (src/flow.F90:72 keeps Rho, Ux, Uy, Uz as four
@@ -48,6 +51,14 @@ struct alignas(16) SweepWin {
   uint32_t pad;
 };
 
+// Coefficients of the update for one combination of first-order flags
+// (bit 0: x, 1: y, 2: z): (C1, C2) of each axis and the axis' term of the
+// denominator, as src/solver.F90:206-214 selects them.
+struct alignas(16) SweepCoef {
+  double x1, x2, y1, y2, z1, z2;
+  double dx, dy, dz, pad;
+};
+
 // Device state that carries from one iteration to the next, in one precision.
 template <typename T>
 struct State {
@@ -65,6 +76,11 @@ struct Ctx {
   int nc_fund = 0;
   std::vector<double> cx, cy, cz, w;
   double mu = 0, diff_flux = 0, mass = 1.0;
+  // several right-hand sides: Kn, mu and the stored mass of each (rhs 0 is
+  // the single solve's)
+  int nrhs = 1;
+  std::vector<double> kn_r, mu_r, mass_r;
+  double* d_mu = nullptr;
   int exact = 0;               // serial reductions in the reference's order
   hipStream_t stream = nullptr;
   bool own_stream = true;      // false: the stream belongs to the caller
@@ -93,7 +109,10 @@ struct Ctx {
   bool xcd_remap = true;       // contiguous task ranges per XCD
   int sweep_block = 64;        // threads per sweep workgroup (one wave: a task's
                                // workgroup ends with its wave, measured fastest)
-  bool sweep_vload = false;    // task records and moments through the vector memory path
+  bool sweep_recip = false;    // fast mode: multiply by a tabulated 1 / denominator
+  bool sweep_generic = false;  // single solve through the multi-right-hand-side kernel
+  double* inv_den = nullptr;   // [first-order flags][velocity]
+  SweepCoef* coef = nullptr;   // [first-order flags]
   int32_t* hp_task = nullptr;
   SweepWin* hp_win = nullptr;  // storage window per (hyperplane, octant)
   void* view_args[2] = {nullptr, nullptr};  // per precision, device copies for the sweep
@@ -121,15 +140,17 @@ size_t ctx_device_bytes(const Ctx& c);
 int ctx_create_device(Ctx& c, hipStream_t stream);
 // Enqueue nsteps iterations (no synchronisation, no timing).
 int ctx_enqueue(Ctx& c, int nsteps);
-// Enqueue the copy of Ux on the plane x = 2 (Ny*Nz doubles, j fastest) into
-// host memory (pinned, so the copy is asynchronous).
+// Enqueue the copy of Ux on the plane x = 2 (Ny*Nz doubles per right-hand
+// side, j fastest) into host memory (pinned, so the copy is asynchronous).
 int ctx_enqueue_plane(Ctx& c, double* host_plane);
-// Mass flow through x = 2 from that plane, in the reference's summation order
-// (massLocal of src/output.F90:80-142).
+// Mass flow through x = 2 from one right-hand side's plane, in the
+// reference's summation order (massLocal of src/output.F90:80-142).
 double ctx_mass(const Ctx& c, const double* plane);
-// chkConverge's update of the stored mass (src/output.F90:151-155):
-// out = {mass, permeability, error}.
-void ctx_check(Ctx& c, double m, double out[3]);
+// chkConverge's update of the stored mass of right-hand side r
+// (src/output.F90:151-155): out = {mass, permeability, error}.
+void ctx_check(Ctx& c, int r, double m, double out[3]);
+// Set the right-hand sides before ctx_create_device: nrhs Knudsen numbers.
+int ctx_set_rhs(Ctx& c, int nrhs, const double* kn);
 // Switch the carried state's precision (synchronises the stream).
 int ctx_set_phase(Ctx& c, int phase);
 // Free everything (stream-ordered frees on a caller's stream do not block).

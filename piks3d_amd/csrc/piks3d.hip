@@ -105,8 +105,14 @@ struct SweepArgs {
   const double* cy;
   const double* cz;
   const double* w;
-  double mu, c0, c1, c2, d1;  // 1.5/ds, 2/ds, -0.5/ds, 1/ds
+  const double* mu;        // per right-hand side: sqrt(pi/2) / Kn
+  double mu0;              // mu of right-hand side 0 (the only one of a single solve)
+  const double* inv_den;   // [first-order flags][velocity] 1 / denominator, right-hand side 0
+  const SweepCoef* coef;   // [first-order flags]
+  double c0, c1, c2, d1;   // 1.5/ds, 2/ds, -0.5/ds, 1/ds
   const ViewArgs<T>* view; // what the rare switched-wall reads need
+  int64_t rs_f, rs_mom;    // strides between right-hand sides (elements)
+  int32_t nrhs;
   int32_t ntask;
   int32_t Nc8;
   int32_t xcd_remap;
@@ -129,14 +135,14 @@ struct ViewArgs {
 // o, when the reference would have switched that wall slot by then; null if
 // the plain f slot applies.
 template <typename T>
-__device__ __forceinline__ const T* switched_source(const ViewArgs<T>& a, int64_t n, int o,
-                                                    int axis) {
+__device__ __forceinline__ const T* switched_source(const ViewArgs<T>& a, const T* f, const T* fw, int64_t n,
+                                                    int o, int axis) {
   if (a.lab[n] < 20) return nullptr;
   const int w = a.wid[n];
   const int src = (int)((a.views[w] >> (8 * o + (axis == 2 ? 4 : 0))) & 15u);
   if (src == 0) return nullptr;
-  if (src <= 8) return a.f + fat(a.d, src - 1, n);
-  return a.fw + ((size_t)a.wall_corner[w] * 8 + o) * a.d.Nc8;
+  if (src <= 8) return f + fat(a.d, src - 1, n);
+  return fw + ((size_t)a.wall_corner[w] * 8 + o) * a.d.Nc8;
 }
 
 // Sum of four per-lane values over the wave in 7 exchanges: lane 0 ends with
@@ -199,14 +205,16 @@ struct ViewFix {
 };
 
 template <typename T>
-__device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell, int o, int ll) {
+__device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell, int o, int ll, int rhs) {
   const ViewArgs<T>& a = *va;
+  const T* f = a.f + rhs * a.d.rs_f;
+  const T* fw = a.fw + rhs * a.d.rs_fw;
   const int64_t dy = (int64_t)osy(o) * a.d.Nxt, dz = (int64_t)osz(o) * a.d.Nxyt;
   const int64_t nbr[4] = {(int64_t)cell - dy, (int64_t)cell - 2 * dy, (int64_t)cell - dz, (int64_t)cell - 2 * dz};
   ViewFix<T> r{};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const T* src = switched_source(a, nbr[q], o, q < 2 ? 1 : 2);
+    const T* src = switched_source(a, f, fw, nbr[q], o, q < 2 ? 1 : 2);
     if (src) {
       r.v[q] = src[ll];
       r.mask |= 1u << q;
@@ -226,7 +234,7 @@ __device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell
 // Each update is src/solver.F90:206-214 (and analogues) with the same
 // operations in the same order (contraction off): signs enter as exact
 // negations, and a first-order axis contributes (c*0)*f like the reference.
-template <typename T, int kThreads, bool kVecLoad>
+template <typename T, int kThreads, bool kRecip, int RB>
 __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
   // the update runs in FP64 in both phases: FP32 arithmetic leaves a larger
   // error along the iteration's slowest mode than FP32 storage alone
@@ -237,86 +245,162 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
   const uint32_t t = xcd_block(a.xcd_remap) * (kThreads / kWave) + wave;
   if (t >= (uint32_t)a.ntask) return;
   const int lane = threadIdx.x & (kWave - 1);
-  // the record and the moments come either through the scalar cache or
-  // through the vector memory path (lanes 0-7 load the record's words, lanes
-  // 0-3 the moments, broadcast with readlane)
-  TaskRec r;
-  if (kVecLoad) {
-    const int32_t rw = lane < 8 ? reinterpret_cast<const int32_t*>(a.tasks + t)[lane] : 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r.w[q] = __builtin_amdgcn_readlane(rw, q);
-  } else {
-    r = ld_const(a.tasks + t);
-  }
+  const TaskRec r = ld_const(a.tasks + t);
   const int o = (int)((uint32_t)r.w[0] >> 28);
   const uint32_t self = ((uint32_t)r.w[0] & 0x0fffffffu) * (uint32_t)sizeof(T);
   const uint32_t cell = (uint32_t)r.w[7] & 0x0fffffffu;
   const uint32_t flg = (uint32_t)r.w[7] >> 28;
-  Mom m;
-  if (!kVecLoad) {
-    m = ld_const(a.mom + cell);
-  } else {
-    const double mvl = lane < 4 ? reinterpret_cast<const double*>(a.mom + cell)[lane] : 0.0;
-    const int64_t bits = __builtin_bit_cast(int64_t, mvl);
-    const int lo = (int)(bits & 0xffffffff), hi = (int)(bits >> 32);
-    auto pick = [&](int q) {
-      const uint32_t l32 = (uint32_t)__builtin_amdgcn_readlane(lo, q);
-      const uint32_t h32 = (uint32_t)__builtin_amdgcn_readlane(hi, q);
-      return __builtin_bit_cast(double, ((uint64_t)h32 << 32) | l32);
-    };
-    m.r = pick(0); m.x = pick(1); m.y = pick(2); m.z = pick(3);
-  }
   const SweepWin win = ld_const(a.win + o);
-  const __amdgpu_buffer_rsrc_t rs =
-      __builtin_amdgcn_make_buffer_rsrc(a.f + win.off, 0, (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
+  // this wave's right-hand sides: RB of them from r0 (the record is shared)
+  const int r0 = blockIdx.y * RB;
+  const int nr = a.nrhs - r0 < RB ? a.nrhs - r0 : RB;
+  Mom m[RB];
+  C mu[RB];
+  __amdgpu_buffer_rsrc_t rs[RB];
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const int rr = k < nr ? r0 + k : r0;
+    m[k] = ld_const(a.mom + rr * a.rs_mom + cell);
+    mu[k] = a.nrhs == 1 ? a.mu0 : ld_const(a.mu + rr);
+    rs[k] = __builtin_amdgcn_make_buffer_rsrc(a.f + rr * a.rs_f + win.off, 0,
+                                              (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
+  }
   const int sx = osx(o), sy = osy(o), sz = osz(o);
   const int Nc8 = a.Nc8;
-  const C mx = (C)m.x, my = (C)m.y, mz = (C)m.z, mr = (C)m.r, mu = (C)a.mu;
+  const bool fx = flg & 1u, fy = flg & 2u, fz = flg & 4u;
   for (int base = 0; base < Nc8; base += kWave) {
     const int l = base + lane;
     const bool lact = l < Nc8;
     const int ll = lact ? l : 0;
     const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
-    const T fp = buf_load<T>(rs, voff, self);
-    T nb[6];
+    T fp[RB], nb[RB][6];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      fp[k] = buf_load<T>(rs[k], voff, self);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) nb[k][q] = buf_load<T>(rs[k], voff, (uint32_t)r.w[1 + q] * (uint32_t)sizeof(T));
+    }
+    const C cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
+    const C d1 = a.d1, c0 = a.c0, c1 = a.c1, c2 = a.c2;
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      if (k >= nr) break;
+      if (flg & 8u) {  // reads a wall slot the reference switched
+        const ViewFix<T> vf = view_fix(a.view, cell, o, ll, r0 + k);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (vf.mask & (1u << q)) nb[k][2 + q] = vf.v[q];
+      }
+      C tt = sx > 0 ? cxl * m[k].x : -(cxl * m[k].x);
+      tt = sy > 0 ? tt + cyl * m[k].y : tt - cyl * m[k].y;
+      tt = sz > 0 ? tt + czl * m[k].z : tt - czl * m[k].z;
+      const C feq = wl * (m[k].r + C(1) * tt);
+      C num = mu[k] * (feq - C(0.5) * fp[k]);
+      num = num + cxl * (fx ? d1 : c1) * nb[k][0];
+      num = num + cxl * (fx ? C(0) : c2) * nb[k][1];
+      num = num + cyl * (fy ? d1 : c1) * nb[k][2];
+      num = num + cyl * (fy ? C(0) : c2) * nb[k][3];
+      num = num + czl * (fz ? d1 : c1) * nb[k][4];
+      num = num + czl * (fz ? C(0) : c2) * nb[k][5];
+      C fnew;
+      if (kRecip) {  // 1 / denominator from the table (single right-hand side)
+        fnew = num * a.inv_den[(flg & 7u) * (uint32_t)Nc8 + (uint32_t)ll];
+      } else {
+        C den = C(0.5) * mu[k] + cxl * (fx ? d1 : c0);
+        den = den + cyl * (fy ? d1 : c0);
+        den = den + czl * (fz ? d1 : c0);
+        fnew = num / den;
+      }
+      if (lact) buf_store((T)fnew, rs[k], voff, self);
+      if (a.part) {
+        // moments are summed in FP64 in both phases
+        const double fv = lact ? (double)(T)fnew : 0.0;
+        const double z = wave_sum4<double>(fv, cxl * fv, cyl * fv, czl * fv, lane);
+        if ((lane & 15) == 0) {
+          const int q = lane >> 4;
+          const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
+          double* pp = a.part + (size_t)(r0 + k) * a.rs_mom * 32 + ((size_t)cell * 8 + o) * 4 + q;
+          // velocity passes after the first add to the first pass's sum
+          *pp = base == 0 ? sgn * z : *pp + sgn * z;
+        }
+      }
+    }
+  }
+}
+
+// The single-solve sweep with the least scalar work per task: one task per
+// wave, the flag-dependent coefficients read from a table, the octant's signs
+// applied to the moments (an exact negation) instead of branching.  The
+// per-wave instruction count bounds this kernel, so every scalar operation on
+// its path counts.
+template <typename T, bool kRecip>
+__global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
+  const uint32_t t = xcd_block(a.xcd_remap);
+  if (t >= (uint32_t)a.ntask) return;
+  const int lane = threadIdx.x;
+  const TaskRec r = ld_const(a.tasks + t);
+  const uint32_t o = (uint32_t)r.w[0] >> 28;
+  const uint32_t self = ((uint32_t)r.w[0] & 0x0fffffffu) * (uint32_t)sizeof(T);
+  const uint32_t cell = (uint32_t)r.w[7] & 0x0fffffffu;
+  const uint32_t flg = (uint32_t)r.w[7] >> 28;
+  const SweepWin win = ld_const(a.win + o);
+  const Mom m = ld_const(a.mom + cell);
+  const SweepCoef cf = ld_const(a.coef + (flg & 7u));
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(a.f + win.off, 0, (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
+  const int Nc8 = a.Nc8;
+  // signed moments: cx*(-u) is -(cx*u) exactly, and t - y is t + (-y)
+  const double mx = osx((int)o) > 0 ? m.x : -m.x;
+  const double my = osy((int)o) > 0 ? m.y : -m.y;
+  const double mz = osz((int)o) > 0 ? m.z : -m.z;
+  for (int base = 0; base < Nc8; base += kWave) {
+    const int l = base + lane;
+    const bool lact = l < Nc8;
+    const int ll = lact ? l : 0;
+    const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
+    const double fp = buf_load<T>(rs, voff, self);
+    double nb[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) nb[q] = buf_load<T>(rs, voff, (uint32_t)r.w[1 + q] * (uint32_t)sizeof(T));
+    const double kinv = kRecip ? a.inv_den[(flg & 7u) * (uint32_t)Nc8 + (uint32_t)ll] : 0.0;
     if (flg & 8u) {  // reads a wall slot the reference switched
-      const ViewFix<T> vf = view_fix(a.view, cell, o, ll);
+      const ViewFix<T> vf = view_fix(a.view, cell, (int)o, ll, 0);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if (vf.mask & (1u << q)) nb[2 + q] = vf.v[q];
     }
-    const C cxl = (C)a.cx[ll], cyl = (C)a.cy[ll], czl = (C)a.cz[ll], wl = (C)a.w[ll];
-    const bool fx = flg & 1u, fy = flg & 2u, fz = flg & 4u;
-    const C d1 = (C)a.d1, c0 = (C)a.c0, c1 = (C)a.c1, c2 = (C)a.c2;
-    C tt = sx > 0 ? cxl * mx : -(cxl * mx);
-    tt = sy > 0 ? tt + cyl * my : tt - cyl * my;
-    tt = sz > 0 ? tt + czl * mz : tt - czl * mz;
-    const C feq = wl * (mr + C(1) * tt);
-    C num = mu * (feq - C(0.5) * fp);
-    num = num + cxl * (fx ? d1 : c1) * nb[0];
-    num = num + cxl * (fx ? C(0) : c2) * nb[1];
-    num = num + cyl * (fy ? d1 : c1) * nb[2];
-    num = num + cyl * (fy ? C(0) : c2) * nb[3];
-    num = num + czl * (fz ? d1 : c1) * nb[4];
-    num = num + czl * (fz ? C(0) : c2) * nb[5];
-    C den = C(0.5) * mu + cxl * (fx ? d1 : c0);
-    den = den + cyl * (fy ? d1 : c0);
-    den = den + czl * (fz ? d1 : c0);
-    const C fnew = num / den;
+    const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
+    double tt = cxl * mx;
+    tt = tt + cyl * my;
+    tt = tt + czl * mz;
+    const double feq = wl * (m.r + 1.0 * tt);
+    double num = a.mu0 * (feq - 0.5 * fp);
+    num = num + cxl * cf.x1 * nb[0];
+    num = num + cxl * cf.x2 * nb[1];
+    num = num + cyl * cf.y1 * nb[2];
+    num = num + cyl * cf.y2 * nb[3];
+    num = num + czl * cf.z1 * nb[4];
+    num = num + czl * cf.z2 * nb[5];
+    double fnew;
+    if (kRecip) {
+      fnew = num * kinv;
+    } else {
+      double den = 0.5 * a.mu0 + cxl * cf.dx;
+      den = den + cyl * cf.dy;
+      den = den + czl * cf.dz;
+      fnew = num / den;
+    }
     if (lact) buf_store((T)fnew, rs, voff, self);
     if (a.part) {
-      // moments are summed in FP64 in both phases: the mass balance they carry
-      // into the next iteration is what the slowest mode of the iteration feels
       const double fv = lact ? (double)(T)fnew : 0.0;
       const double z = wave_sum4<double>(fv, cxl * fv, cyl * fv, czl * fv, lane);
       if ((lane & 15) == 0) {
+        // the signs of the Ux, Uy, Uz partials are the octant's
         const int q = lane >> 4;
-        const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
+        const double zz = q == 1 ? (osx((int)o) > 0 ? z : -z) : q == 2 ? (osy((int)o) > 0 ? z : -z)
+                        : q == 3 ? (osz((int)o) > 0 ? z : -z) : z;
         double* pp = a.part + ((size_t)cell * 8 + o) * 4 + q;
-        // velocity passes after the first add to the first pass's sum
-        *pp = base == 0 ? sgn * z : *pp + sgn * z;
+        *pp = base == 0 ? zz : *pp + zz;
       }
     }
   }
@@ -332,6 +416,8 @@ __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
                          const uint64_t* views, Dims d) {
   const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (idx >= nwalls) return;
+  f += blockIdx.y * d.rs_f;
+  fw += blockIdx.y * d.rs_fw;
   const int lane = threadIdx.x & (kWave - 1);
   const int w = walls[idx];
   const int64_t cell = wall_cell[w];
@@ -355,6 +441,9 @@ __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
 template <typename T>
 __global__ void k_halo(T* f, T* buf_e, T* buf_w, Dims d) {
   const int64_t n = (int64_t)d.Ny * d.Nz * 8 * d.Nc8;
+  f += blockIdx.y * d.rs_f;
+  buf_e += blockIdx.y * d.rs_buf;
+  buf_w += blockIdx.y * d.rs_buf;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -511,6 +600,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
   const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (idx >= a.n) return;
+  a.f += blockIdx.y * a.d.rs_f;
+  a.fw += blockIdx.y * a.d.rs_fw;
+  a.snap += blockIdx.y * a.d.rs_snap;
   wall_dispatch<T, false>(a, idx, threadIdx.x & (kWave - 1));
 }
 
@@ -521,12 +613,17 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_walls_coupled(WallArgs<T> a) {
   const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (idx >= a.n) return;
+  a.f += blockIdx.y * a.d.rs_f;
+  a.fw += blockIdx.y * a.d.rs_fw;
+  a.snap += blockIdx.y * a.d.rs_snap;
   wall_dispatch<T, true>(a, idx, threadIdx.x & (kWave - 1));
 }
 
 template <typename T>
 __global__ void k_snapshot(const T* f, T* snap, const int32_t* cells, int n, Dims d) {
   const int64_t tot = (int64_t)n * 8 * d.Nc8;
+  f += blockIdx.y * d.rs_f;
+  snap += blockIdx.y * d.rs_snap;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -543,6 +640,7 @@ __global__ void k_snapshot(const T* f, T* snap, const int32_t* cells, int n, Dim
 template <typename T>
 __global__ void k_inout(T* f, const double* w, double pd, Dims d) {
   const int64_t n = (int64_t)d.Ny * d.Nz * 4 * d.Nc8;
+  f += blockIdx.y * d.rs_f;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -569,6 +667,7 @@ __device__ __forceinline__ void copy_slot(T* f, const Dims& d, int64_t cell, int
 template <typename T>
 __global__ void k_sym_y(T* f, Dims d) {
   const int64_t n = (int64_t)d.Nx * d.Nz * d.Nc8;
+  f += blockIdx.y * d.rs_f;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -587,6 +686,7 @@ __global__ void k_sym_y(T* f, Dims d) {
 template <typename T>
 __global__ void k_sym_z(T* f, Dims d) {
   const int64_t n = (int64_t)d.Nx * d.Ny * d.Nc8;
+  f += blockIdx.y * d.rs_f;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -605,7 +705,10 @@ __global__ void k_sym_z(T* f, Dims d) {
 
 // Interior fluid cells: their f is final after the sweep, so the moments are
 // the sum of the eight octant partials.
-__global__ void k_moments_partials(const double* part, const int32_t* cells, int64_t n, Mom* mom) {
+__global__ void k_moments_partials(const double* part, const int32_t* cells, int64_t n, Mom* mom,
+                                   int64_t ntotal) {
+  part += blockIdx.y * ntotal * 32;
+  mom += blockIdx.y * ntotal;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = cells[t];
@@ -626,6 +729,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_moments_direct(const T* f, const int32_t* cells, int64_t n,
                                                         const double* cx, const double* cy,
                                                         const double* cz, Dims d, Mom* mom) {
+  f += blockIdx.y * d.rs_f;
+  mom += blockIdx.y * d.Ntotal;
   const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (idx >= n) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -650,6 +755,8 @@ __global__ __launch_bounds__(256) void k_moments_direct(const T* f, const int32_
 template <typename T>
 __global__ void k_moments_serial(const T* f, const int32_t* cells, int64_t n, const double* cx,
                                  const double* cy, const double* cz, Dims d, Mom* mom) {
+  f += blockIdx.y * d.rs_f;
+  mom += blockIdx.y * d.Ntotal;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t c = cells[t];
@@ -669,6 +776,8 @@ __global__ void k_moments_serial(const T* f, const int32_t* cells, int64_t n, co
 
 __global__ void k_gather_plane(const Mom* mom, double* out, Dims d, int column) {
   const int64_t n = (int64_t)d.Ny * d.Nz;
+  mom += blockIdx.y * d.Ntotal;
+  out += blockIdx.y * n;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int j = 1 + (int)(t % d.Ny), k = 1 + (int)(t / d.Ny);
@@ -755,18 +864,21 @@ WallArgs<T> wall_args(Ctx& c, State<T>& st, const int32_t* list, int n) {
 template <typename T>
 using SweepKernel = void (*)(SweepArgs<T>);
 
-template <typename T, bool V>
-SweepKernel<T> sweep_kernel_v(int block) {
-  switch (block) {
-    case 128: return k_sweep<T, 128, V>;
-    case 256: return k_sweep<T, 256, V>;
-    default: return k_sweep<T, 64, V>;
-  }
+template <typename T, int RB>
+SweepKernel<T> sweep_kernel_rb(int block) {
+  return block == 256 ? k_sweep<T, 256, false, RB> : k_sweep<T, 64, false, RB>;
 }
 
+// RB right-hand sides per wave (sharing the task record); with `recip` a
+// single right-hand side multiplies by the tabulated 1 / denominator
 template <typename T>
-SweepKernel<T> sweep_kernel(int block, bool vec_load) {
-  return vec_load ? sweep_kernel_v<T, true>(block) : sweep_kernel_v<T, false>(block);
+SweepKernel<T> sweep_kernel(int block, bool recip, int rb) {
+  if (recip && rb == 1) return block == 256 ? k_sweep<T, 256, true, 1> : k_sweep<T, 64, true, 1>;
+  switch (rb) {
+    case 2: return sweep_kernel_rb<T, 2>(block);
+    case 4: return sweep_kernel_rb<T, 4>(block);
+    default: return sweep_kernel_rb<T, 1>(block);
+  }
 }
 
 // Enqueue nsteps iterations on the context's stream.  With `timed`, four
@@ -781,11 +893,19 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
   sa.mom = c.mom;
   sa.part = c.exact ? nullptr : c.part;
   sa.cx = c.dcx; sa.cy = c.dcy; sa.cz = c.dcz; sa.w = c.dw;
-  sa.mu = c.mu;
+  sa.mu = c.d_mu;
+  sa.mu0 = c.mu_r[0];
+  sa.inv_den = c.inv_den;
+  sa.coef = c.coef;
   sa.c0 = 1.5 / g.ds; sa.c1 = 2.0 / g.ds; sa.c2 = -0.5 / g.ds; sa.d1 = 1.0 / g.ds;
   sa.view = static_cast<const ViewArgs<T>*>(c.view_args[sizeof(T) == sizeof(double) ? 0 : 1]);
   sa.Nc8 = d.Nc8;
   sa.xcd_remap = c.xcd_remap;
+  sa.nrhs = c.nrhs;
+  sa.rs_f = d.rs_f;
+  sa.rs_mom = d.Ntotal;
+  const int rb = c.nrhs >= 4 ? 4 : c.nrhs >= 2 ? 2 : 1;
+  const unsigned ny_sweep = (unsigned)((c.nrhs + rb - 1) / rb), R = (unsigned)c.nrhs;
   const int wpb = 4;  // waves per block for one-wave-per-item kernels
   const int timed = timed_run ? std::min(nsteps, kMaxTimedSteps) : 0;
   while ((int)c.ev.size() < 4 * timed) {
@@ -803,50 +923,57 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
       sa.win = c.hp_win + (size_t)h * 8;
       const int64_t wpg = c.sweep_block / kWave;
       const int64_t blocks = ((sa.ntask + wpg - 1) / wpg + 7) / 8 * 8;
-      hipLaunchKernelGGL(sweep_kernel<T>(c.sweep_block, c.sweep_vload), dim3((unsigned)blocks),
+      if (c.nrhs == 1 && c.sweep_block == 64 && !c.sweep_generic) {
+        const bool recip = c.sweep_recip && !c.exact;
+        SweepKernel<T> k1 = recip ? k_sweep1<T, true> : k_sweep1<T, false>;
+        hipLaunchKernelGGL(k1, dim3((unsigned)blocks), dim3(64), 0, c.stream, sa);
+        continue;
+      }
+      hipLaunchKernelGGL(sweep_kernel<T>(c.sweep_block, c.sweep_recip && !c.exact, rb),
+                         dim3((unsigned)blocks, ny_sweep),
                          dim3(c.sweep_block), 0, c.stream, sa);
     }
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
     if (!g.commit_walls.empty())
-      hipLaunchKernelGGL(k_commit<T>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb)),
+      hipLaunchKernelGGL(k_commit<T>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb), R),
                          dim3(wpb * kWave), 0, c.stream, st.f, (const T*)st.fw, c.commit_walls,
                          (int)g.commit_walls.size(), c.wall_cell, c.wall_corner, c.views, d);
-    hipLaunchKernelGGL(k_halo<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256)), dim3(256),
+    hipLaunchKernelGGL(k_halo<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256), R), dim3(256),
                        0, c.stream, st.f, st.buf_e, st.buf_w, d);
     if (!g.snap_cell.empty())
-      hipLaunchKernelGGL(k_snapshot<T>, dim3(grid_for((int64_t)g.snap_cell.size() * 8 * d.Nc8, 256)),
+      hipLaunchKernelGGL(k_snapshot<T>, dim3(grid_for((int64_t)g.snap_cell.size() * 8 * d.Nc8, 256), R),
                          dim3(256), 0, c.stream, (const T*)st.f, st.snap, c.snap_cell,
                          (int)g.snap_cell.size(), d);
     if (!g.uncoupled.empty()) {
       WallArgs<T> wa = wall_args<T>(c, st, c.uncoupled, (int)g.uncoupled.size());
-      hipLaunchKernelGGL(k_walls<T>, dim3(grid_for((int64_t)g.uncoupled.size(), wpb)),
+      hipLaunchKernelGGL(k_walls<T>, dim3(grid_for((int64_t)g.uncoupled.size(), wpb), R),
                          dim3(wpb * kWave), 0, c.stream, wa);
     }
     for (size_t lv = 0; lv + 1 < g.coupled_level_off.size(); ++lv) {
       const int b = g.coupled_level_off[lv], n = g.coupled_level_off[lv + 1] - b;
       WallArgs<T> wa = wall_args<T>(c, st, c.coupled + b, n);
       wa.snap_idx = c.coupled_snap + (size_t)b * 6;
-      hipLaunchKernelGGL(k_walls_coupled<T>, dim3(grid_for(n, wpb)), dim3(wpb * kWave), 0,
+      hipLaunchKernelGGL(k_walls_coupled<T>, dim3(grid_for(n, wpb), R), dim3(wpb * kWave), 0,
                          c.stream, wa);
     }
-    hipLaunchKernelGGL(k_inout<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256)), dim3(256),
+    hipLaunchKernelGGL(k_inout<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256), R), dim3(256),
                        0, c.stream, st.f, (const double*)c.dw, c.prm.press_drop, d);
-    hipLaunchKernelGGL(k_sym_y<T>, dim3(grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_sym_y<T>, dim3(grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256), R), dim3(256), 0,
                        c.stream, st.f, d);
-    hipLaunchKernelGGL(k_sym_z<T>, dim3(grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_sym_z<T>, dim3(grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256), R), dim3(256), 0,
                        c.stream, st.f, d);
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 2], c.stream));
     if (c.exact) {
       const int64_t n = (int64_t)g.fluid_interior.size() + (int64_t)g.fluid_boundary.size();
-      hipLaunchKernelGGL(k_moments_serial<T>, dim3(grid_for(n, 64)), dim3(64), 0, c.stream,
+      hipLaunchKernelGGL(k_moments_serial<T>, dim3(grid_for(n, 64), R), dim3(64), 0, c.stream,
                          (const T*)st.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, c.mom);
     } else {
       const int64_t ni = (int64_t)g.fluid_interior.size(), nbd = (int64_t)g.fluid_boundary.size();
       if (ni > 0)
-        hipLaunchKernelGGL(k_moments_partials, dim3(grid_for(ni, 256)), dim3(256), 0, c.stream,
-                           (const double*)c.part, c.fluid_int, ni, c.mom);
+        hipLaunchKernelGGL(k_moments_partials, dim3(grid_for(ni, 256), R), dim3(256), 0, c.stream,
+                           (const double*)c.part, c.fluid_int, ni, c.mom, g.Ntotal);
       if (nbd > 0)
-        hipLaunchKernelGGL(k_moments_direct<T>, dim3((unsigned)((nbd + wpb - 1) / wpb)),
+        hipLaunchKernelGGL(k_moments_direct<T>, dim3((unsigned)((nbd + wpb - 1) / wpb), R),
                            dim3(wpb * kWave), 0, c.stream, (const T*)st.f, c.fluid_bnd, nbd,
                            c.dcx, c.dcy, c.dcz, d, c.mom);
     }
@@ -912,13 +1039,21 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   if (const char* b = std::getenv("PIKS_XCD_REMAP")) c->xcd_remap = std::atoi(b) != 0;
   if (const char* b = std::getenv("PIKS_SWEEP_BLOCK")) {
     const int v = std::atoi(b);
-    if (v != 64 && v != 128 && v != 256) return fail(PIKS_E_ARG, "PIKS_SWEEP_BLOCK must be 64, 128 or 256");
+    if (v != 64 && v != 256) return fail(PIKS_E_ARG, "PIKS_SWEEP_BLOCK must be 64 or 256");
     c->sweep_block = v;
   }
 
   if (const char* b = std::getenv("PIKS_GRAPH")) c->use_graph = std::atoi(b) != 0;
-  if (const char* b = std::getenv("PIKS_SWEEP_VLOAD")) c->sweep_vload = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_SWEEP_RECIP")) c->sweep_recip = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_SWEEP_GENERIC")) c->sweep_generic = std::atoi(b) != 0;
   c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
+  c->kn_r.assign(1, p->kn);
+  c->mu_r.assign(1, c->mu);
+  c->mass_r.assign(1, 1.0);
+  c->d.rs_f = (int64_t)c->n_f;
+  c->d.rs_fw = (int64_t)c->n_fw;
+  c->d.rs_buf = (int64_t)c->n_buf;
+  c->d.rs_snap = (int64_t)c->n_snap;
   // the sweep addresses one hyperplane's storage window with 32-bit offsets
   int64_t maxwin = 0;
   for (int32_t v : g.hp_len) maxwin = std::max<int64_t>(maxwin, v);
@@ -944,14 +1079,15 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
 size_t ctx_device_bytes(const Ctx& c) {
   const Geometry& g = c.g;
   const bool refined = c.prm.precision == PIKS_PREC_FP32_REFINED;
-  const size_t state64 = (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(double);
-  const size_t state32 = refined ? (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(float) : 0;
-  const size_t per_cell = (size_t)g.Ntotal * (sizeof(Mom) + 32 * sizeof(double) + 1 + 4 + 4 * 4);
+  const size_t R = (size_t)c.nrhs;
+  const size_t state64 = R * (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(double);
+  const size_t state32 = refined ? R * (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(float) : 0;
+  const size_t per_cell = (size_t)g.Ntotal * (R * (sizeof(Mom) + 32 * sizeof(double)) + 1 + 4 + 4 * 4);
   const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.uncoupled.size() * 4 +
                        g.coupled.size() * 4 * 7 + g.snap_cell.size() * 4 + g.commit_walls.size() * 4;
   const size_t fluid = (size_t)(g.fluid_interior.size() + g.fluid_boundary.size()) * 8;
   return state64 + state32 + per_cell + walls + fluid + g.hp_task.size() * 4 + g.hp_lo.size() * 8 +
-         (size_t)g.Ny * g.Nz * 8 + 4 * c.cx.size() * 8 + 64 * 256;
+         R * ((size_t)g.Ny * g.Nz * 8 + 8) + 4 * c.cx.size() * 8 + 64 * 256;
 }
 
 int ctx_create_device(Ctx& c, hipStream_t stream) {
@@ -973,20 +1109,45 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
   const Geometry& g = c.g;
   const bool refined = c.prm.precision == PIKS_PREC_FP32_REFINED;
   int rc = 0;
-  if ((rc = dalloc(c, &c.s64.f, c.n_f)) || (rc = dalloc(c, &c.s64.fw, c.n_fw)) ||
-      (rc = dalloc(c, &c.s64.buf_e, c.n_buf)) || (rc = dalloc(c, &c.s64.buf_w, c.n_buf)) ||
-      (rc = dalloc(c, &c.s64.snap, c.n_snap)))
+  const size_t R = (size_t)c.nrhs;
+  if ((rc = dalloc(c, &c.s64.f, R * c.n_f)) || (rc = dalloc(c, &c.s64.fw, R * c.n_fw)) ||
+      (rc = dalloc(c, &c.s64.buf_e, R * c.n_buf)) || (rc = dalloc(c, &c.s64.buf_w, R * c.n_buf)) ||
+      (rc = dalloc(c, &c.s64.snap, R * c.n_snap)))
     return rc;
   if (refined) {
-    if ((rc = dalloc(c, &c.s32.f, c.n_f)) || (rc = dalloc(c, &c.s32.fw, c.n_fw)) ||
-        (rc = dalloc(c, &c.s32.buf_e, c.n_buf)) || (rc = dalloc(c, &c.s32.buf_w, c.n_buf)) ||
-        (rc = dalloc(c, &c.s32.snap, c.n_snap)))
+    if ((rc = dalloc(c, &c.s32.f, R * c.n_f)) || (rc = dalloc(c, &c.s32.fw, R * c.n_fw)) ||
+        (rc = dalloc(c, &c.s32.buf_e, R * c.n_buf)) || (rc = dalloc(c, &c.s32.buf_w, R * c.n_buf)) ||
+        (rc = dalloc(c, &c.s32.snap, R * c.n_snap)))
       return rc;
     c.phase = PIKS_PREC_FP32_REFINED;
   }
-  if ((rc = dalloc(c, &c.mom, (size_t)g.Ntotal))) return rc;
-  if ((rc = dalloc(c, &c.part, (size_t)g.Ntotal * 32))) return rc;
-  if ((rc = dalloc(c, &c.plane, (size_t)g.Ny * g.Nz))) return rc;
+  if ((rc = dalloc(c, &c.mom, R * (size_t)g.Ntotal))) return rc;
+  if ((rc = dalloc(c, &c.part, R * (size_t)g.Ntotal * 32))) return rc;
+  if ((rc = dalloc(c, &c.plane, R * (size_t)g.Ny * g.Nz))) return rc;
+  if ((rc = upload(c, &c.d_mu, c.mu_r))) return rc;
+  {
+    // 1 / denominator of the update (src/solver.F90:206-214) per first-order
+    // flags and velocity, for right-hand side 0
+    const double c0 = 1.5 / g.ds, d1 = 1.0 / g.ds;
+    const int nc8 = c.d.Nc8;
+    std::vector<double> inv((size_t)8 * nc8);
+    for (int fl = 0; fl < 8; ++fl)
+      for (int l = 0; l < nc8; ++l) {
+        double den = 0.5 * c.mu_r[0] + c.cx[(size_t)l] * ((fl & 1) ? d1 : c0);
+        den = den + c.cy[(size_t)l] * ((fl & 2) ? d1 : c0);
+        den = den + c.cz[(size_t)l] * ((fl & 4) ? d1 : c0);
+        inv[(size_t)fl * nc8 + l] = 1.0 / den;
+      }
+    if ((rc = upload(c, &c.inv_den, inv))) return rc;
+    const double c1 = 2.0 / g.ds, c2 = -0.5 / g.ds;
+    std::vector<SweepCoef> cf(8);
+    for (int fl = 0; fl < 8; ++fl) {
+      const bool fx = fl & 1, fy = fl & 2, fz = fl & 4;
+      cf[(size_t)fl] = SweepCoef{fx ? d1 : c1, fx ? 0.0 : c2, fy ? d1 : c1, fy ? 0.0 : c2,
+                                 fz ? d1 : c1, fz ? 0.0 : c2, fx ? d1 : c0, fy ? d1 : c0, fz ? d1 : c0, 0.0};
+    }
+    if ((rc = upload(c, &c.coef, cf))) return rc;
+  }
   if ((rc = upload(c, &c.dcx, c.cx)) || (rc = upload(c, &c.dcy, c.cy)) ||
       (rc = upload(c, &c.dcz, c.cz)) || (rc = upload(c, &c.dw, c.w)))
     return rc;
@@ -1066,11 +1227,11 @@ int ctx_enqueue(Ctx& c, int nsteps) {
 int ctx_enqueue_plane(Ctx& c, double* host_plane) {
   const Geometry& g = c.g;
   HIP_TRY(hipSetDevice(c.device));
-  hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)g.Ny * g.Nz, 256)), dim3(256), 0, c.stream,
+  hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)g.Ny * g.Nz, 256), (unsigned)c.nrhs), dim3(256), 0, c.stream,
                      (const Mom*)c.mom, c.plane, c.d, 2);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(host_plane, c.plane, (size_t)g.Ny * g.Nz * sizeof(double), hipMemcpyDeviceToHost,
-                         c.stream));
+  HIP_TRY(hipMemcpyAsync(host_plane, c.plane, (size_t)g.Ny * g.Nz * c.nrhs * sizeof(double),
+                         hipMemcpyDeviceToHost, c.stream));
   return 0;
 }
 
@@ -1093,12 +1254,28 @@ double ctx_mass(const Ctx& c, const double* u) {
          c.prm.press_drop / (rl * rl);
 }
 
-void ctx_check(Ctx& c, double m, double out[3]) {
-  const double error = std::fabs(1.0 - m / c.mass) / (double)c.prm.chk_converge_step;
-  c.mass = m;
+void ctx_check(Ctx& c, int r, double m, double out[3]) {
+  const double error = std::fabs(1.0 - m / c.mass_r[(size_t)r]) / (double)c.prm.chk_converge_step;
+  c.mass_r[(size_t)r] = m;
+  if (r == 0) c.mass = m;
   out[0] = m;
-  out[1] = m * c.prm.kn * std::sqrt(4.0 / (std::atan(1.0) * 4.0));
+  out[1] = m * c.kn_r[(size_t)r] * std::sqrt(4.0 / (std::atan(1.0) * 4.0));
   out[2] = error;
+}
+
+int ctx_set_rhs(Ctx& c, int nrhs, const double* kn) {
+  if (nrhs < 1 || nrhs > 1024) return fail(PIKS_E_ARG, "number of right-hand sides must be 1..1024");
+  if (c.s64.f) return fail(PIKS_E_STATE, "right-hand sides are set before the device set-up");
+  c.nrhs = nrhs;
+  c.kn_r.assign(kn, kn + nrhs);
+  c.mu_r.resize((size_t)nrhs);
+  c.mass_r.assign((size_t)nrhs, 1.0);
+  for (int r = 0; r < nrhs; ++r) {
+    if (!(kn[r] > 0.0)) return fail(PIKS_E_ARG, "Kn must be positive");
+    c.mu_r[(size_t)r] = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / kn[r];   // src/flow.F90:90
+  }
+  c.mu = c.mu_r[0];
+  return 0;
 }
 
 int ctx_set_phase(Ctx& c, int phase) {
@@ -1108,15 +1285,15 @@ int ctx_set_phase(Ctx& c, int phase) {
   HIP_TRY(hipSetDevice(c.device));
   const int blocks = 4096;
   if (phase == PIKS_PREC_FP64) {
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.f, (const float*)c.s32.f, c.n_f);
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.fw, (const float*)c.s32.fw, c.n_fw);
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_e, (const float*)c.s32.buf_e, c.n_buf);
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_w, (const float*)c.s32.buf_w, c.n_buf);
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.f, (const float*)c.s32.f, c.n_f * c.nrhs);
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.fw, (const float*)c.s32.fw, c.n_fw * c.nrhs);
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_e, (const float*)c.s32.buf_e, c.n_buf * c.nrhs);
+    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_w, (const float*)c.s32.buf_w, c.n_buf * c.nrhs);
   } else {
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.f, (const double*)c.s64.f, c.n_f);
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.fw, (const double*)c.s64.fw, c.n_fw);
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_e, (const double*)c.s64.buf_e, c.n_buf);
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_w, (const double*)c.s64.buf_w, c.n_buf);
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.f, (const double*)c.s64.f, c.n_f * c.nrhs);
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.fw, (const double*)c.s64.fw, c.n_fw * c.nrhs);
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_e, (const double*)c.s64.buf_e, c.n_buf * c.nrhs);
+    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_w, (const double*)c.s64.buf_w, c.n_buf * c.nrhs);
   }
   HIP_TRY(hipGetLastError());
   if (c.own_stream) HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1179,16 +1356,65 @@ int piks_set_precision(void* ctx, int32_t precision) {
 }
 
 static int mass_flow_sync(Ctx& c, double* m) {
-  std::vector<double> u((size_t)c.g.Ny * c.g.Nz);
+  const size_t np = (size_t)c.g.Ny * c.g.Nz;
+  std::vector<double> u(np * c.nrhs);
   if (int rc = ctx_enqueue_plane(c, u.data())) return rc;
   HIP_TRY(hipStreamSynchronize(c.stream));
-  *m = ctx_mass(c, u.data());
+  for (int r = 0; r < c.nrhs; ++r) m[r] = ctx_mass(c, u.data() + np * r);
   return 0;
 }
 
 int piks_mass_flow(void* ctx, double* m) {
   if (!ctx || !m) return fail(PIKS_E_ARG, "null argument");
-  return mass_flow_sync(*static_cast<Ctx*>(ctx), m);
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  std::vector<double> all((size_t)c.nrhs);
+  if (int rc = mass_flow_sync(c, all.data())) return rc;
+  *m = all[0];
+  return 0;
+}
+
+int piks_create_multi(const piks_params* p, const uint8_t* raw_image, int64_t image_bytes, int32_t nrhs,
+                      const double* kn, void** ctx_out) {
+  g_err.clear();
+  if (!ctx_out || !kn) return fail(PIKS_E_ARG, "null argument");
+  *ctx_out = nullptr;
+  Ctx* c = nullptr;
+  if (int rc = ctx_create_host(p, raw_image, image_bytes, &c)) return rc;
+  int rc = ctx_set_rhs(*c, nrhs, kn);
+  if (rc == 0) rc = ctx_create_device(*c, nullptr);
+  if (rc) {
+    ctx_destroy(c);
+    return rc;
+  }
+  *ctx_out = c;
+  return 0;
+}
+
+int piks_chk_converge_multi(void* ctx, double* out) {
+  if (!ctx || !out) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  std::vector<double> m((size_t)c.nrhs);
+  if (int rc = mass_flow_sync(c, m.data())) return rc;
+  for (int r = 0; r < c.nrhs; ++r) ctx_check(c, r, m[(size_t)r], out + 3 * r);
+  return 0;
+}
+
+int piks_get_macro_rhs(void* ctx, int32_t rhs, double* rho, double* ux, double* uy, double* uz) {
+  if (!ctx) return fail(PIKS_E_ARG, "null context");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  if (rhs < 0 || rhs >= c.nrhs) return fail(PIKS_E_ARG, "right-hand side out of range");
+  HIP_TRY(hipSetDevice(c.device));
+  std::vector<Mom> h((size_t)c.g.Ntotal);
+  HIP_TRY(hipMemcpyAsync(h.data(), c.mom + (size_t)rhs * c.g.Ntotal, h.size() * sizeof(Mom),
+                         hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
+  for (size_t q = 0; q < h.size(); ++q) {
+    if (rho) rho[q] = h[q].r;
+    if (ux) ux[q] = h[q].x;
+    if (uy) uy[q] = h[q].y;
+    if (uz) uz[q] = h[q].z;
+  }
+  return 0;
 }
 
 int piks_chk_converge(void* ctx, double out[3]) {
@@ -1196,7 +1422,7 @@ int piks_chk_converge(void* ctx, double out[3]) {
   Ctx& c = *static_cast<Ctx*>(ctx);
   double m = 0.0;
   if (int rc = mass_flow_sync(c, &m)) return rc;
-  ctx_check(c, m, out);  // src/output.F90:151-155
+  ctx_check(c, 0, m, out);  // src/output.F90:151-155
   return 0;
 }
 

@@ -164,7 +164,7 @@ void on_chunk_done(Server& s, Slot& sl, Request& r) {
   bool done = false;
   if (r.step % chk == 0) {
     double out[3];
-    ctx_check(c, ctx_mass(c, sl.plane), out);
+    ctx_check(c, 0, ctx_mass(c, sl.plane), out);
     r.mass = out[0];
     r.perm = out[1];
     r.error = out[2];

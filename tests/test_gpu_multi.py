@@ -1,0 +1,50 @@
+"""Several Knudsen numbers on one geometry (piks_create_multi).  Each
+right-hand side runs the same kernels with its own mu and state, so its
+iterates must equal those of a single-Kn context bit for bit; the oracle
+pins the single-Kn context."""
+import numpy as np
+import pytest
+
+from conftest import load_golden, params_from_golden
+from piks3d_amd import MultiSolver, Solver, images
+from piks3d_amd.params import Params
+
+pytestmark = pytest.mark.gpu
+
+KNS = [0.3, 1.0, 5.0, 0.07, 2.5]
+
+
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("name", ["boxes_n2_a05_e1", "coupled_fl0_a06"])
+def test_each_rhs_equals_its_single_solve(name, exact):
+    z = load_golden(name)
+    p = params_from_golden(z)
+    img = z["image"]
+    with MultiSolver(p, img, KNS, exact_order=exact) as ms:
+        ms.iterate(5)
+        multi = [ms.macro_rhs(r) for r in range(len(KNS))]
+        m_all, k_all, e_all = ms.chk_converge_all()
+    for r, kn in enumerate(KNS):
+        p.kn = kn
+        with Solver(p, img, exact_order=exact) as s:
+            s.iterate(5)
+            single = s.macro()
+            m, k, e = s.chk_converge()
+        for a, b in zip(multi[r], single):
+            assert np.array_equal(a, b), (name, kn)
+        assert (m_all[r], k_all[r], e_all[r]) == (m, k, e)
+
+
+def test_refined_multi_matches_fp64_multi():
+    n = 32
+    img = images.sphere_pack(n, 0.40, 3.0, 6.0, seed=9)
+    p = Params(nx=n, ny=n, nz=n, fluid_layer=2, wall_ext_order=2, nc_fundamental=4, half_range=True,
+               chk_converge_step=10, max_step=500, eps=0.0, kn=1.0, press_drop=1.0, accom=1.0)
+    kns = [0.5, 2.0, 8.0]
+    with MultiSolver(p, img, kns, precision="fp64") as a:
+        r64 = a.solve()
+    with MultiSolver(p, img, kns, precision="fp32") as b:
+        r32 = b.solve()
+    assert 0 < r32["steps32"] < r32["steps"] == r64["steps"] == 500
+    rel = np.abs(np.asarray(r32["permeability"]) - r64["permeability"]) / np.abs(r64["permeability"])
+    assert np.all(rel <= 1e-8), rel
