@@ -114,6 +114,8 @@ def serve_workload(nreq, seed, max_step, eps):
                    max_step=max_step, eps=eps, kn=kn, press_drop=1.0, accom=1.0)
         _, info = preprocess(p, img)
         reqs.append((p, img, info.nfluid * 8 * nc ** 3))
+        if (q + 1) % 16 == 0:
+            log("generated %d of %d" % (q + 1, nreq))
     return reqs
 
 
@@ -133,11 +135,13 @@ def serve_bench(args):
             if delay > 0:
                 time.sleep(delay)
             tickets.append(srv.submit(p, img, precision=args.precision))
+            if len(tickets) % 32 == 0:
+                log("%d submitted, %.1f s" % (len(tickets), time.perf_counter() - t0))
         log("all %d submitted after %.1f s" % (len(tickets), time.perf_counter() - t0))
         res = []
         for q, t in enumerate(tickets):
             res.append(srv.wait(t))
-            if (q + 1) % 32 == 0:
+            if (q + 1) % 8 == 0:
                 log("%d done, %.1f s" % (q + 1, time.perf_counter() - t0))
         wall = time.perf_counter() - t0
     ok = [r for r in res if r["status"] == REQ_DONE]
@@ -159,6 +163,86 @@ def serve_bench(args):
     print(json.dumps(out))
 
 
+def kn_sweep_bench(args):
+    """Multi-RHS Knudsen sweep: one 160^3 Gaussian-random-field image
+    (correlation length 5 voxels, porosity 0.25, percolating in x), 32 Kn
+    log-spaced over [1e-3, 10], 8^3 velocities, solved as groups of right-hand
+    sides sharing the geometry (one sweep launch per hyperplane for a group),
+    FP32 iterations with the FP64 finish; a few Kn are re-solved in plain FP64
+    for the same number of iterations and compared."""
+    import numpy as np
+    from piks3d_amd import MultiSolver, images
+    from piks3d_amd.params import Params
+    img = images.gaussian_field(160, 0.25, 5.0, seed=args.seed)
+    p = Params(nx=160, ny=160, nz=160, fluid_layer=4, wall_ext_order=2, nc_fundamental=4, half_range=True,
+               chk_converge_step=10, max_step=args.max_step, eps=args.eps, kn=1.0, press_drop=1.0, accom=1.0)
+    kns = np.logspace(-3, 1, args.nkn)
+    perm, steps, conv = np.zeros(len(kns)), np.zeros(len(kns), int), np.zeros(len(kns), bool)
+    updates, wall = 0.0, 0.0
+    for g0 in range(0, len(kns), args.group):
+        ks = list(kns[g0:g0 + args.group])
+        t = time.perf_counter()
+        with MultiSolver(p, img, ks, precision=args.precision) as ms:
+            nf = ms.info.nfluid
+            r = ms.solve()
+        dt = time.perf_counter() - t
+        wall += dt
+        updates += nf * 512 * r["steps"] * len(ks)
+        perm[g0:g0 + len(ks)] = r["permeability"]
+        steps[g0:g0 + len(ks)] = r["steps"]
+        conv[g0:g0 + len(ks)] = np.asarray(r["error"]) <= args.eps
+        log("Kn %.3g..%.3g: %d steps (%d FP32), %.1f s" % (ks[0], ks[-1], r["steps"], r["steps32"], dt))
+    check = [0, len(kns) // 3, 2 * len(kns) // 3, len(kns) - 1]
+    rel = []
+    for q in check:
+        with MultiSolver(p, img, [kns[q]], precision="fp64") as ms:
+            r64 = ms.solve(max_step=int(steps[q]), eps=0.0)
+        rel.append(abs(perm[q] - r64["permeability"][0]) / abs(r64["permeability"][0]))
+    out = {
+        "metric": "multi-RHS Knudsen sweep: cell*velocity updates/s", "value": updates / wall,
+        "unit": "updates/s", "higher_is_better": True, "n_gpus": 1, "wall_s": wall,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "kn": [float(k) for k in kns], "permeability": [float(k) for k in perm],
+        "steps": [int(x) for x in steps], "converged": [bool(x) for x in conv],
+        "check_vs_fp64": {"kn": [float(kns[q]) for q in check], "rel_diff": rel},
+        "config": {"workload": "grf160_phi025_l5_kn32", "group": args.group, "max_step": args.max_step,
+                   "eps": args.eps, "precision": args.precision, "seed": args.seed},
+    }
+    print(json.dumps(out))
+
+
+def large_bench(args):
+    """The large single solve: a 300^3 Gaussian-random-field image
+    (correlation length 6, porosity 0.20, percolating in x), 8^3 velocities,
+    Kn in {0.01, 1, 10}, FP64 and FP32 storage: sweep throughput over a few
+    iterations each."""
+    from piks3d_amd import Solver, images
+    from piks3d_amd.params import Params
+    img = images.gaussian_field(300, 0.20, 6.0, seed=args.seed)
+    rows = []
+    for kn in (0.01, 1.0, 10.0):
+        p = Params(nx=300, ny=300, nz=300, fluid_layer=4, wall_ext_order=2, nc_fundamental=4, half_range=True,
+                   chk_converge_step=10, max_step=2, kn=kn, press_drop=1.0, accom=1.0)
+        for prec in ("fp64", "fp32"):
+            with Solver(p, img, precision=prec) as s:
+                s.iterate(args.warmup)
+                t = time.perf_counter()
+                s.iterate(args.steps)
+                dt = time.perf_counter() - t
+                ms = s.timing()
+                upd = s.info.nfluid * 512 * args.steps
+                rows.append({"kn": kn, "precision": prec, "updates_per_s": upd / dt,
+                             "sweep_ms_per_step": ms[0] / args.steps, "ms_per_step": dt / args.steps * 1e3,
+                             "device_gb": s.info.device_bytes / 1e9, "fluid_cells": s.info.nfluid})
+                log("Kn %g %s: %.3g updates/s, %.1f ms/step, %.1f GB" % (kn, prec, upd / dt, dt / args.steps * 1e3,
+                                                                           s.info.device_bytes / 1e9))
+    print(json.dumps({"metric": "large single solve: cell*velocity updates/s", "unit": "updates/s",
+                      "value": min(r["updates_per_s"] for r in rows if r["precision"] == "fp64"),
+                      "higher_is_better": True, "n_gpus": 1, "data": "synthetic", "rows": rows,
+                      "config": {"workload": "grf300_phi020_l6", "steps": args.steps, "warmup": args.warmup,
+                                 "seed": args.seed}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +256,12 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--serve", action="store_true",
                     help="run the serving stream instead (Poisson arrivals into the request server)")
+    ap.add_argument("--kn-sweep", action="store_true",
+                    help="run the multi-RHS Knudsen sweep on a 160^3 random field instead")
+    ap.add_argument("--large", action="store_true",
+                    help="run the 300^3 random-field single solve at Kn 0.01, 1, 10 instead")
+    ap.add_argument("--nkn", type=int, default=32)
+    ap.add_argument("--group", type=int, default=16)
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=8.0, help="serving: mean arrivals per second")
     ap.add_argument("--slots", type=int, default=8)
@@ -196,6 +286,12 @@ def main():
         return
     if args.serve:
         serve_bench(args)
+        return
+    if args.kn_sweep:
+        kn_sweep_bench(args)
+        return
+    if args.large:
+        large_bench(args)
         return
 
     try:

@@ -174,6 +174,12 @@ def preprocess(params: Params, image):
     return labels, info
 
 
+def fp64_tail(max_step, chk):
+    """Iterations at the end of a refined solve's budget that always run in
+    FP64: the larger of three checks and a fifth of max_step."""
+    return max(3 * chk, max_step // 5)
+
+
 class Solver:
     """One DVM solve on one GPU (see module docstring)."""
 
@@ -262,9 +268,10 @@ class Solver:
         steps, permeability, mass, error and the check history.
 
         A context created with precision 'fp32' runs FP32 iterations while
-        the check's error is above refine_switch (and still falling), then
-        switches to FP64 for the remaining iterations; `steps32` in the result
-        counts the FP32 ones."""
+        the check's error is above refine_switch, still falling, and the
+        iteration budget's FP64 tail (fp64_tail) has not begun, then switches
+        to FP64 for the remaining iterations; `steps32` in the result counts
+        the FP32 ones."""
         p = self.params
         max_step = p.max_step if max_step is None else max_step
         chk = p.chk_converge_step if chk_step is None else chk_step
@@ -298,7 +305,7 @@ class Solver:
                         best, stalled = error, 0
                     else:
                         stalled += 1
-                    if error <= max(tau, eps) or stalled >= 3:
+                    if error <= max(tau, eps) or stalled >= 3 or step >= max_step - fp64_tail(max_step, chk):
                         self.set_precision("fp64")
                         refined = False
                         error = 1.0
@@ -440,7 +447,7 @@ class MultiSolver(Solver):
                 improved = error < best
                 best = np.where(improved, error, best)
                 stalled = np.where(improved, 0, stalled + 1)
-                if np.all((error <= max(tau, eps)) | (stalled >= 3)):
+                if np.all((error <= max(tau, eps)) | (stalled >= 3)) or step >= max_step - fp64_tail(max_step, chk):
                     self.set_precision("fp64")
                     refined = False
                 continue

@@ -177,7 +177,9 @@ void on_chunk_done(Server& s, Slot& sl, Request& r) {
       } else {
         ++r.stalled;
       }
-      if (r.error <= std::max(r.refine_switch, r.eps) || r.stalled >= 3) {
+      // the last max(3 checks, max_step / 5) iterations always run in FP64
+      const int tail = std::max(3 * chk, r.max_step / 5);
+      if (r.error <= std::max(r.refine_switch, r.eps) || r.stalled >= 3 || r.step >= r.max_step - tail) {
         if (int rc = ctx_set_phase(c, PIKS_PREC_FP64)) return fail_request(s, sl, r, rc);
         r.refined = false;
         r.error = 1.0;

@@ -15,8 +15,9 @@
 !     &gpuNml precision = 'fp64' | 'fp32', device = 0, exact_quadrature = F,
 !             refineSwitch = 1.0d-6 /
 ! With precision = 'fp32' the iterations run in FP32 until the check's error
-! is below refineSwitch or has stopped falling for three checks; the rest run
-! in FP64, so the converged permeability is the FP64 solve's.
+! is below refineSwitch, has stopped falling for three checks, or the last
+! max(3 checks, maxStep / 5) iterations begin; the rest run in FP64, so the
+! converged permeability is the FP64 solve's.
 program piks3d_gpu
     use iso_c_binding
     use piks_c
@@ -164,7 +165,9 @@ program piks3d_gpu
             else
                 stalled = stalled + 1
             endif
-            if (error <= max(refineSwitch, eps) .or. stalled >= 3) then
+            ! the last max(3 checks, maxStep / 5) iterations always run in FP64
+            if (error <= max(refineSwitch, eps) .or. stalled >= 3 .or. &
+                iStep >= maxStep - max(3 * chkConvergeStep, maxStep / 5)) then
                 rc = piks_set_precision(ctx, PIKS_PREC_FP64)
                 call check_rc(rc)
                 refined = .false.

@@ -74,6 +74,30 @@ module piks_c
             type(c_ptr), value :: ctx, rho, ux, uy, uz
         end function piks_get_macro
 
+        integer(c_int) function piks_create_multi(p, raw_image, image_bytes, nrhs, kn, ctx) &
+                bind(C, name='piks_create_multi')
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, piks_params
+            type(piks_params), intent(in) :: p
+            type(c_ptr), value :: raw_image
+            integer(c_int64_t), value :: image_bytes
+            integer(c_int32_t), value :: nrhs
+            real(c_double), intent(in) :: kn(*)
+            type(c_ptr), intent(out) :: ctx
+        end function piks_create_multi
+
+        integer(c_int) function piks_chk_converge_multi(ctx, out) bind(C, name='piks_chk_converge_multi')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out(3, *)
+        end function piks_chk_converge_multi
+
+        integer(c_int) function piks_get_macro_rhs(ctx, rhs, rho, ux, uy, uz) bind(C, name='piks_get_macro_rhs')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: rhs
+            type(c_ptr), value :: rho, ux, uy, uz
+        end function piks_get_macro_rhs
+
         integer(c_int) function piks_get_labels(ctx, labels) bind(C, name='piks_get_labels')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, labels
