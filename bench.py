@@ -333,8 +333,10 @@ def main():
 
     wl = "spheres200_phi040_kn1_v8"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(wl),
-            "kernel": "k_sweep", "bytes_per_update": bytes_per_update,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(wl if args.precision == "fp64" else wl + "_fp32"),
+            "kernel": "k_sweep1<double, false>" if args.precision == "fp64" else "k_sweep1<float, false>",
+            "bytes_per_update": bytes_per_update,
             "sweep_ms_per_step": round(ms[0] / args.steps, 4)}
     cpu = None
     if not args.no_cpu_baseline:
