@@ -243,6 +243,69 @@ def large_bench(args):
                                  "seed": args.seed}}))
 
 
+# The reference's converged permeability for the 64^3 sphere configuration
+# (its own run stops at iteration 80 with K = 1.86415031246455858e-02; the
+# value is also held by tests/golden/c1_sphere64.npz).
+C1_K_REFERENCE = 1.86415031246455858e-02
+
+
+def c1_bench(args):
+    """The CPU-runnable configuration to convergence: 64^3 image with one
+    centred solid sphere of radius 28, 4 fluid layers at inlet and outlet,
+    8^3 half-range velocities, Kn 0.1, unit pressure drop, check every 10
+    iterations, eps 1e-6.  GPU (FP64, both reduction modes) time to solution
+    beside the reference program run to convergence on the host cores."""
+    from piks3d_amd import Solver, images
+    from piks3d_amd.params import Params, format_para
+    img = images.centered_sphere(64, 28.0)
+    p = Params(image_file_name="sphere64.raw", nx=64, ny=64, nz=64, fluid_layer=4, wall_ext_order=2,
+               nc_fundamental=4, half_range=True, chk_converge_step=10, max_step=100000, eps=1e-6,
+               kn=0.1, press_drop=1.0, accom=1.0)
+    rows = {}
+    for exact in (False, True):
+        t = time.perf_counter()
+        with Solver(p, img, precision="fp64", exact_order=exact) as s:
+            t1 = time.perf_counter()
+            r = s.solve()
+            t2 = time.perf_counter()
+            nf, nc = s.info.nfluid, s.info.nc8 * 8
+        rows["exact" if exact else "fast"] = {
+            "steps": r["steps"], "permeability": r["permeability"],
+            "rel_diff_vs_reference": abs(r["permeability"] - C1_K_REFERENCE) / C1_K_REFERENCE,
+            "setup_s": t1 - t, "solve_s": t2 - t1, "updates_per_s": nf * nc * r["steps"] / (t2 - t1)}
+        log("C1 %s: %d steps, K %.17e, solve %.3f s" % ("exact" if exact else "fast", r["steps"],
+                                                       r["permeability"], t2 - t1))
+    cpu = None
+    exe = os.path.join(HERE, "oracle", "_ref", "piks3d")
+    if not args.no_cpu_baseline and os.path.exists(exe):
+        with tempfile.TemporaryDirectory(prefix="piks_ref_") as d:
+            images.write_raw(os.path.join(d, p.image_file_name), img)
+            pr = Params(**dict(p.__dict__))
+            pr.save_last = False
+            pr.save_step = 1 << 30
+            with open(os.path.join(d, "para.in"), "w") as fh:
+                fh.write(format_para(pr))
+            env = dict(os.environ, OMP_NUM_THREADS=str(args.cpu_threads))
+            log("C1 reference program to convergence, %d threads" % args.cpu_threads)
+            r = subprocess.run([exe], cwd=d, env=env, capture_output=True, text=True, timeout=900)
+            m = re.search(r"Walltime=\s*([0-9.Ee+-]+),\s*Steps=\s*([0-9]+)", r.stdout)
+            if r.returncode == 0 and m:
+                wall, steps = float(m.group(1)), int(m.group(2))
+                cpu = {"value": wall, "unit": "s", "cores": args.cpu_threads, "kind": "reference",
+                       "steps": steps, "updates_per_s": nf * nc * steps / wall,
+                       "sample": "the whole run to convergence, reference PIKS3D (flang -O2 -fopenmp, "
+                                 "1 MPI rank)"}
+            else:
+                log("reference run failed:", r.stdout[-500:], r.stderr[-500:])
+    fast = rows["fast"]
+    print(json.dumps({
+        "metric": "time to converged permeability, CPU-runnable configuration", "value": fast["solve_s"],
+        "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+        "updates_per_s": fast["updates_per_s"], "rows": rows, "cpu_baseline": cpu,
+        "config": {"workload": "sphere64_r28_kn01_v8", "fluid_cells": nf, "velocities": nc,
+                   "eps": p.eps, "chk_converge_step": p.chk_converge_step}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,6 +321,8 @@ def main():
                     help="run the serving stream instead (Poisson arrivals into the request server)")
     ap.add_argument("--kn-sweep", action="store_true",
                     help="run the multi-RHS Knudsen sweep on a 160^3 random field instead")
+    ap.add_argument("--c1", action="store_true",
+                    help="the 64^3 sphere configuration to convergence, GPU and reference program")
     ap.add_argument("--large", action="store_true",
                     help="run the 300^3 random-field single solve at Kn 0.01, 1, 10 instead")
     ap.add_argument("--nkn", type=int, default=32)
@@ -292,6 +357,9 @@ def main():
         return
     if args.large:
         large_bench(args)
+        return
+    if args.c1:
+        c1_bench(args)
         return
 
     try:
