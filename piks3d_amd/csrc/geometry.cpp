@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <climits>
 #include <array>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 
@@ -11,6 +12,22 @@
 #include "wallprog.h"
 
 namespace piks {
+
+namespace {
+thread_local int tl_host_threads = 0;
+}
+
+int host_threads() {
+  if (tl_host_threads > 0) return tl_host_threads;
+  if (const char* e = std::getenv("PIKS_HOST_THREADS")) {
+    const int n = std::atoi(e);
+    if (n > 0) return std::min(n, 64);
+  }
+  const unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hw, 16u));
+}
+
+void set_host_threads(int n) { tl_host_threads = n > 0 ? n : 0; }
 
 namespace {
 
@@ -80,6 +97,7 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
     err = "domain too small: need Nx >= 1, Ny >= 3, Nz >= 3";
     return PIKS_E_ARG;
   }
+  const int threads = in.threads > 0 ? in.threads : host_threads();
   const int fl = in.fluid_layer;
   g.Nx = in.nx + 2 * fl;
   g.Ny = in.ny;
@@ -125,7 +143,8 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
 
   // labels (src/physicalGrid.F90:511-671)
   g.lab.assign((size_t)g.Ntotal, GHOST);
-  for (int k = 1; k <= g.Nz; ++k)
+  parallel_for(g.Nz, threads, [&](int kk) {
+    const int k = kk + 1;
     for (int j = 1; j <= g.Ny; ++j)
       for (int i = 1; i <= g.Nx; ++i) {
         int8_t v = G.at(i, j, k);
@@ -143,6 +162,7 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
         }
         g.lab[(size_t)g.cell(i, j, k)] = (uint8_t)lab;
       }
+  });
 
   // walls in raster order, 3-fold corner ids (src/physicalGrid.F90:701-748)
   g.wid.assign((size_t)g.Ntotal, -1);
@@ -223,7 +243,10 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
   // raster order makes x readers see the unswitched slot, y readers the
   // y-switched one, z readers the z-switched one, else the y-switched one.
   g.wall_views.assign(nwall, 0);
-  for (size_t w = 0; w < nwall; ++w) {
+  std::vector<uint8_t> commits(nwall, 0);
+  const int kWallChunk = 4096;
+  parallel_for((int)((nwall + kWallChunk - 1) / kWallChunk), threads, [&](int chunk) {
+  for (size_t w = (size_t)chunk * kWallChunk; w < std::min(nwall, (size_t)(chunk + 1) * kWallChunk); ++w) {
     int64_t c = g.wall_cell[w];
     int lab = g.wall_label[w];
     uint64_t v = 0;
@@ -238,8 +261,11 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
       any = any || zc != 0;
     }
     g.wall_views[w] = v;
-    if (any) g.commit_walls.push_back((int32_t)w);
+    commits[w] = any;
   }
+  });
+  for (size_t w = 0; w < nwall; ++w)
+    if (commits[w]) g.commit_walls.push_back((int32_t)w);
 
   // per fluid cell: first-order flags and the switched-wall flag per octant
   // (src/physicalGrid.F90:1254-1622: first order along an axis when the cell
@@ -247,14 +273,19 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
   g.code.assign((size_t)g.Ntotal, 0);
   const int face_toward[3][2] = {{kSideE, kSideW}, {kSideN, kSideS}, {kSideF, kSideB}};
   const int64_t stride[3] = {1, g.Nxt, g.Nxyt};
-  for (int k = 1; k <= g.Nz; ++k)
+  // one z plane per work item; the fluid lists are joined in plane order, so
+  // they keep the raster order of a serial pass
+  std::vector<std::vector<int32_t>> plane_int((size_t)g.Nz + 1), plane_bnd((size_t)g.Nz + 1);
+  parallel_for(g.Nz, threads, [&](int kk) {
+    const int k = kk + 1;
+    std::vector<int32_t>& l_int = plane_int[(size_t)k];
+    std::vector<int32_t>& l_bnd = plane_bnd[(size_t)k];
     for (int j = 1; j <= g.Ny; ++j)
       for (int i = 1; i <= g.Nx; ++i) {
         int64_t c = g.cell(i, j, k);
         if (g.lab[(size_t)c] != FLUID) continue;
-        ++g.nfluid;
         bool bnd = i == 1 || i == g.Nx || j == 1 || j == g.Ny || k == 1 || k == g.Nz;
-        (bnd ? g.fluid_boundary : g.fluid_interior).push_back((int32_t)c);
+        (bnd ? l_bnd : l_int).push_back((int32_t)c);
         uint32_t code = 0;
         for (int o = 0; o < 8; ++o) {
           int s[3] = {osx(o), osy(o), osz(o)};
@@ -279,6 +310,12 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
         }
         g.code[(size_t)c] = code;
       }
+  });
+  for (int k = 1; k <= g.Nz; ++k) {
+    g.fluid_interior.insert(g.fluid_interior.end(), plane_int[(size_t)k].begin(), plane_int[(size_t)k].end());
+    g.fluid_boundary.insert(g.fluid_boundary.end(), plane_bnd[(size_t)k].begin(), plane_bnd[(size_t)k].end());
+  }
+  g.nfluid = (int64_t)(g.fluid_interior.size() + g.fluid_boundary.size());
 
   // walls whose extrapolation reads another wall (see geometry.h)
   std::vector<int> level(nwall, 0);
@@ -333,11 +370,11 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
     err = "domain of " + std::to_string(g.Ntotal) + " cells (with ghosts) is too large";
     return PIKS_E_ARG;
   }
-  build_sweep_schedule(g);
+  build_sweep_schedule(g, threads);
   return PIKS_OK;
 }
 
-void build_sweep_schedule(Geometry& g) {
+void build_sweep_schedule(Geometry& g, int threads) {
   // cells that are stored
   std::vector<uint8_t> stored((size_t)g.Ntotal, 0);
   for (int k = 1; k <= g.Nz; ++k)
@@ -352,7 +389,7 @@ void build_sweep_schedule(Geometry& g) {
   g.nstore = ns;
   // family orders: counting sort on sx*i + sy*j + sz*k, stable over (k, j)
   const int kmin = -(g.Nx + 2) - (g.Ny + 2) - (g.Nz + 2), nkey = 2 * (-kmin) + 1;
-  for (int F = 0; F < 4; ++F) {
+  parallel_for(4, threads, [&](int F) {
     const int sx = osx(F), sy = osy(F), sz = osz(F);
     std::vector<int64_t> start((size_t)nkey + 1, 0);
     for (int k = -1; k <= g.Nz + 2; ++k)
@@ -368,7 +405,7 @@ void build_sweep_schedule(Geometry& g) {
           const int64_t c = g.cell(i, j, k);
           if (stored[(size_t)c]) pos[(size_t)c] = (int32_t)start[(size_t)(sx * i + sy * j + sz * k - kmin)]++;
         }
-  }
+  });
 
   // tasks, bucketed by (hyperplane, octant); raster order (k, j) inside a
   // bucket is ascending storage order
@@ -392,7 +429,8 @@ void build_sweep_schedule(Geometry& g) {
       g.hp_task.assign((size_t)count[(size_t)nb] * W, 0);
       at.assign(count.begin(), count.end() - 1);
     }
-    for (int o = 0; o < 8; ++o) {
+    // the octants fill disjoint buckets (b % 8 == o): one work item each
+    parallel_for(8, threads, [&](int o) {
       const std::vector<int32_t>& pos = g.pos[ofam(o)];
       const int64_t dx = osx(o), dy = (int64_t)osy(o) * g.Nxt, dz = (int64_t)osz(o) * g.Nxyt;
       for (int k = 1; k <= g.Nz; ++k)
@@ -418,12 +456,14 @@ void build_sweep_schedule(Geometry& g) {
             t[6] = pos[(size_t)(c - 2 * dz)];
             t[7] = (int32_t)((uint32_t)c | flags << 28);
           }
-    }
+    });
   }
   // storage windows
   g.hp_lo.assign((size_t)nb, 0);
   g.hp_len.assign((size_t)nb, 0);
-  for (int64_t b = 0; b < nb; ++b) {
+  const int kChunk = 64;
+  parallel_for((int)((nb + kChunk - 1) / kChunk), threads, [&](int chunk) {
+  for (int64_t b = (int64_t)chunk * kChunk; b < std::min<int64_t>(nb, (int64_t)(chunk + 1) * kChunk); ++b) {
     int32_t lo = INT32_MAX, hi = -1;
     for (int64_t t = count[(size_t)b]; t < count[(size_t)b + 1]; ++t) {
       const int32_t* r = &g.hp_task[(size_t)(t * W)];
@@ -439,6 +479,7 @@ void build_sweep_schedule(Geometry& g) {
       g.hp_len[(size_t)b] = hi - lo + 1;
     }
   }
+  });
 }
 
 }  // namespace piks

@@ -12,7 +12,10 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace piks {
@@ -21,7 +24,20 @@ struct GeomInput {
   int nx, ny, nz;           // image size as in para.in (nx before fluid layers)
   int fluid_layer;
   int wall_ext_order;
+  int threads = 0;          // host threads for the preprocessing (0: host_threads())
 };
+
+// Host threads the preprocessing of one image uses: the calling thread's
+// setting if one was made (set_host_threads), else PIKS_HOST_THREADS from the
+// environment, else the machine's hardware threads, at most 16.  Every
+// parallel loop writes disjoint data, so the result does not depend on the
+// count.
+int host_threads();
+void set_host_threads(int n);  // this thread only; 0 restores the default
+
+// Run f(0..n-1) on up to `threads` threads (the calling thread included).
+template <typename F>
+void parallel_for(int n, int threads, F&& f);
 
 struct Geometry {
   int Nx, Ny, Nz;           // after fluid layers
@@ -94,6 +110,24 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
                    std::string& err);
 
 // Fill the storage order (nstore, pos) and the sweep schedule (hp_task, hp_off).
-void build_sweep_schedule(Geometry& g);
+void build_sweep_schedule(Geometry& g, int threads);
+
+template <typename F>
+void parallel_for(int n, int threads, F&& f) {
+  threads = std::max(1, std::min(threads, n));
+  if (threads == 1) {
+    for (int q = 0; q < n; ++q) f(q);
+    return;
+  }
+  std::atomic<int> next{0};
+  auto run = [&] {
+    for (int q; (q = next.fetch_add(1)) < n;) f(q);
+  };
+  std::vector<std::thread> pool;
+  pool.reserve((size_t)threads - 1);
+  for (int t = 1; t < threads; ++t) pool.emplace_back(run);
+  run();
+  for (auto& t : pool) t.join();
+}
 
 }  // namespace piks

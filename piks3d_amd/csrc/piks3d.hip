@@ -1063,7 +1063,7 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   // (hyperplane, octant) window, the zero slot as the window's end
   Geometry& gm = c->g;
   const int64_t nc8 = c->d.Nc8;
-  for (int h = 0; h < gm.nhp; ++h)
+  parallel_for(gm.nhp, host_threads(), [&](int h) {
     for (int64_t t = gm.hp_off[(size_t)h]; t < gm.hp_off[(size_t)h + 1]; ++t) {
       int32_t* w = &gm.hp_task[(size_t)(t * Geometry::kTaskWords)];
       const int o = (int)((uint32_t)w[0] >> 28);
@@ -1072,6 +1072,7 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
       w[0] = (int32_t)((uint32_t)rel((uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
       for (int q = 1; q <= 6; ++q) w[q] = (int32_t)rel(w[q]);
     }
+  });
   *out = c.release();
   return 0;
 }

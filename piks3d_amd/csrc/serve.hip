@@ -98,6 +98,8 @@ void finish(Server& s, Request& r, int status) {
 }
 
 void prep_worker(Server* s) {
+  // the workers preprocess different requests at once: share the host threads
+  set_host_threads(std::max(1, host_threads() / std::max(1, s->prm.prep_threads)));
   for (;;) {
     Request* r = nullptr;
     {

@@ -1,8 +1,10 @@
 """The sweep's launch variants give the default path's results.
 
-A context reads five environment settings at creation (piks3d.hip,
-ctx_create_host): PIKS_XCD_REMAP (XCD-contiguous task ranges), PIKS_SWEEP_BLOCK
-(64 or 256 threads per sweep workgroup), PIKS_GRAPH (one HIP graph per
+A context reads six environment settings at creation (piks3d.hip,
+ctx_create_host; geometry.cpp): PIKS_HOST_THREADS (host threads of the image
+preprocessing, whose loops write disjoint data), PIKS_XCD_REMAP
+(XCD-contiguous task ranges), PIKS_SWEEP_BLOCK (64 or 256 threads per sweep
+workgroup), PIKS_GRAPH (one HIP graph per
 iteration), PIKS_SWEEP_GENERIC (the multi-right-hand-side kernel for a single
 solve) and PIKS_SWEEP_RECIP (fast mode: multiply by a tabulated reciprocal of
 the denominator).  All but the last compute every update with the same
@@ -19,6 +21,8 @@ from conftest import interior_mask, load_golden, params_from_golden
 pytestmark = pytest.mark.gpu
 
 VARIANTS = [
+    {"PIKS_HOST_THREADS": "1"},
+    {"PIKS_HOST_THREADS": "7"},
     {"PIKS_XCD_REMAP": "0"},
     {"PIKS_SWEEP_BLOCK": "256"},
     {"PIKS_GRAPH": "1"},
@@ -31,7 +35,8 @@ STEPS = 4
 
 def _run(p, img, exact, env, monkeypatch):
     from piks3d_amd import Solver
-    for k in ("PIKS_XCD_REMAP", "PIKS_SWEEP_BLOCK", "PIKS_GRAPH", "PIKS_SWEEP_GENERIC", "PIKS_SWEEP_RECIP"):
+    for k in ("PIKS_XCD_REMAP", "PIKS_SWEEP_BLOCK", "PIKS_GRAPH", "PIKS_SWEEP_GENERIC", "PIKS_SWEEP_RECIP",
+              "PIKS_HOST_THREADS"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
