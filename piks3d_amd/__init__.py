@@ -143,7 +143,7 @@ def _check(rc):
         raise PiksError(rc, lib().piks_last_error().decode())
 
 
-def to_cparams(p: Params, precision=None, exact_order=False, device=None) -> PiksParams:
+def to_cparams(p: Params, precision=None, exact_order=None, device=None) -> PiksParams:
     prec = p.precision if precision is None else precision
     prec_code = {"fp64": PIKS_PREC_FP64, "fp32": PIKS_PREC_FP32_REFINED}.get(prec, prec)
     cp = PiksParams()
@@ -158,7 +158,7 @@ def to_cparams(p: Params, precision=None, exact_order=False, device=None) -> Pik
     cp.precision = int(prec_code)
     cp.device = p.device if device is None else device
     cp.exact_quadrature = int(bool(p.exact_quadrature))
-    cp.exact_order = int(bool(exact_order))
+    cp.exact_order = int(bool(p.exact_order if exact_order is None else exact_order))
     return cp
 
 
@@ -183,7 +183,7 @@ def fp64_tail(max_step, chk):
 class Solver:
     """One DVM solve on one GPU (see module docstring)."""
 
-    def __init__(self, params: Params, image, precision=None, exact_order=False, device=None):
+    def __init__(self, params: Params, image, precision=None, exact_order=None, device=None):
         self.params = params
         raw = np.ascontiguousarray(np.asarray(image, dtype=np.uint8).ravel())
         self._cp = to_cparams(params, precision, exact_order, device)
@@ -383,7 +383,7 @@ class MultiSolver(Solver):
     """Several Knudsen numbers on one geometry (piks_create_multi): one
     context, one sweep per hyperplane for all right-hand sides."""
 
-    def __init__(self, params: Params, image, kn_values, precision=None, exact_order=False, device=None):
+    def __init__(self, params: Params, image, kn_values, precision=None, exact_order=None, device=None):
         self.params = params
         self.kn_values = [float(k) for k in kn_values]
         raw = np.ascontiguousarray(np.asarray(image, dtype=np.uint8).ravel())

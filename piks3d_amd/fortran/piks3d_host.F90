@@ -13,7 +13,9 @@
 !
 ! One optional namelist is added, ignored by the reference:
 !     &gpuNml precision = 'fp64' | 'fp32', device = 0, exact_quadrature = F,
-!             refineSwitch = 1.0d-6 /
+!             exact_order = F, refineSwitch = 1.0d-6 /
+! exact_order = T sums the moments and wall fluxes in the reference's order,
+! so every value (and every output file) equals the reference's bit for bit.
 ! With precision = 'fp32' the iterations run in FP32 until the check's error
 ! is below refineSwitch, has stopped falling for three checks, or the last
 ! max(3 checks, maxStep / 5) iterations begin; the rest run in FP64, so the
@@ -36,14 +38,14 @@ program piks3d_gpu
     double precision :: Kn, pressDrop, accom
     character(len=8) :: precision
     integer :: device
-    logical :: exact_quadrature
+    logical :: exact_quadrature, exact_order
     double precision :: refineSwitch
     namelist /physicalNml/ imageFileName, Nx, Ny, Nz, wallExtOrder, fluidlayer, Ref_L
     namelist /velocityNml/ Nc_fundamental, halfRange
     namelist /mpiNml/ mpi_xdim, mpi_ydim, mpi_zdim, block_repx, block_repy, block_repz
     namelist /solverNml/ maxStep, chkConvergeStep, saveStep, eps, saveLast, saveFormat
     namelist /flowNml/ Kn, pressDrop, accom
-    namelist /gpuNml/ precision, device, exact_quadrature, refineSwitch
+    namelist /gpuNml/ precision, device, exact_quadrature, exact_order, refineSwitch
 
     integer, parameter :: PARAFILE = 10, IMAGEFILE = 20
     type(piks_params), target :: prm
@@ -69,6 +71,7 @@ program piks3d_gpu
     precision = 'fp64'
     device = 0
     exact_quadrature = .false.
+    exact_order = .false.
     refineSwitch = 1.0d-6
 
     open(unit=PARAFILE, file='para.in', status='old', iostat=ios)
@@ -123,7 +126,7 @@ program piks3d_gpu
     prm%precision = merge(PIKS_PREC_FP32_REFINED, PIKS_PREC_FP64, trim(precision) == 'fp32')
     prm%device = device
     prm%exact_quadrature = merge(1, 0, exact_quadrature)
-    prm%exact_order = 0
+    prm%exact_order = merge(1, 0, exact_order)
     prm%reserved = 0
     rc = piks_create(prm, c_loc(raw), int(nbytes, c_int64_t), ctx)
     if (rc /= 0) then
@@ -331,7 +334,7 @@ contains
             enddo
         enddo
         write(13,'(A)') '</DataArray>'
-        write(13,'(A)') '<DataArray type="Float32" Name="U" format="ascii" NumberOfComponents="3">'
+        write(13,'(A)') '<DataArray type="Float32" Name="U" format="ascii"  NumberOfComponents="3">'
         do k = 1, info%nz
             do j = 1, info%ny
                 do i = 1, info%nx
@@ -359,7 +362,7 @@ contains
         write(14,'(A)') '<PCellData Scalars="flag Rho" Vectors ="U">'
         write(14,'(A)') '<DataArray type="Int32" Name="flag" format="ascii"/>'
         write(14,'(A)') '<DataArray type="Float32" Name="Rho" format="ascii"/>'
-        write(14,'(A)') '<DataArray type="Float32" Name="U" format="ascii" NumberOfComponents="3"/>'
+        write(14,'(A)') '<DataArray type="Float32" Name="U" format="ascii"  NumberOfComponents="3"/>'
         write(14,'(A)') '</PCellData>'
         write(14,'(A, 6I4, A)') '<Piece Extent="', 0, info%nx, 0, info%ny, 0, info%nz, &
             '" Source="Field_000.vti"/>'

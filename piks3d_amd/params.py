@@ -13,7 +13,8 @@ with the reference's defaults (wallExtOrder = 2, fluidLayer = 0, Ref_L = 1,
 saveLast = T, saveFormat = 1, block_rep* = 1; src/parameters.F90:91-98).  This
 build adds one optional namelist the reference ignores:
 
-    &gpuNml       precision ('fp64' | 'fp32'), device, exact_quadrature, refineSwitch
+    &gpuNml       precision ('fp64' | 'fp32'), device, exact_quadrature, exact_order,
+                  refineSwitch
 
 The mpi* values are accepted and ignored (one GPU solves the whole domain);
 block_rep* other than 1 (the reference's weak-scaling replication) is refused.
@@ -50,6 +51,7 @@ class Params:
     precision: str = "fp64"
     device: int = 0
     exact_quadrature: bool = False
+    exact_order: bool = False
     refine_switch: float = 1.0e-6
 
     # sizes after the fluid layers are added (src/parameters.F90:121)
@@ -81,7 +83,8 @@ _KEYS = {
                   "saveformat": "save_format"},
     "flownml": {"kn": "kn", "pressdrop": "press_drop", "accom": "accom"},
     "gpunml": {"precision": "precision", "device": "device",
-               "exact_quadrature": "exact_quadrature", "refineswitch": "refine_switch"},
+               "exact_quadrature": "exact_quadrature", "exact_order": "exact_order",
+               "refineswitch": "refine_switch"},
 }
 
 
@@ -211,5 +214,6 @@ def format_para(p: Params, gpu: bool = False) -> str:
     if gpu:
         lines += ["", "&gpuNml", "precision = '%s'" % p.precision, "device = %d" % p.device,
                   "exact_quadrature = %s" % _fmt(bool(p.exact_quadrature)),
+                  "exact_order = %s" % _fmt(bool(p.exact_order)),
                   "refineSwitch = %s" % _fmt(float(p.refine_switch)), "/"]
     return "\n".join(lines) + "\n"

@@ -16,7 +16,10 @@ def pytest_configure(config):
 
 
 def golden_names(prefix=""):
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, prefix + "*.npz")))
+    """Golden cases (inputs plus the reference's state); the outputs_* files
+    hold the reference's field files instead and are loaded by name."""
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, prefix + "*.npz"))
+                  if not os.path.basename(p).startswith("outputs_"))
 
 
 def load_golden(name):

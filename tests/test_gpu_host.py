@@ -4,7 +4,8 @@ The reference's stdout for every golden case is stored in its fixture; the
 host, run on the same para.in and image, must print the same convergence-check
 lines (src/output.F90:156: iStep, mass, permeability, error in ES15.6), write
 Results.dat in the reference's format, and write a VTI field whose flags are
-the reference's labels.
+the reference's labels.  With exact_order = T its field files in all three
+formats equal the reference program's own files byte for byte.
 """
 import os
 import re
@@ -61,3 +62,41 @@ def test_fortran_host_prints_the_reference_lines(name, tmp_path):
     lab = z["label"].reshape(nz + 4, ny + 4, nx + 4)[2:-2, 2:-2, 2:-2].ravel()
     assert np.array_equal(flags, lab)
     assert os.path.exists(tmp_path / "Field.pvti")
+
+
+OUTPUTS = "outputs_boxes_n2_a1_e2"
+
+
+@pytest.mark.parametrize("fmt", [1, 2, 3])
+def test_fortran_host_writes_the_reference_files(fmt, tmp_path):
+    """With exact_order = T every value equals the reference's bit for bit, so
+    the field files (VTI + PVTI, Tecplot, legacy VTK) and Results.dat equal
+    the reference program's own files byte for byte (tests/golden/
+    make_outputs.py wrote those from the reference)."""
+    print("case output format %d" % fmt, flush=True)
+    if not os.path.exists(HOST):
+        pytest.fail("piks3d_gpu is not built (python -m piks3d_amd.build)")
+    from piks3d_amd import images
+    from piks3d_amd.params import format_para
+    want = load_golden(OUTPUTS)
+    z = load_golden("boxes_n2_a1_e2")
+    p = params_from_golden(z)
+    p.image_file_name = "img.raw"
+    p.save_step = 1 << 30
+    p.save_last = True
+    p.save_format = fmt
+    p.exact_order = True
+    images.write_raw(str(tmp_path / "img.raw"), z["image"])
+    (tmp_path / "para.in").write_text(format_para(p, gpu=True))
+    r = subprocess.run([HOST], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    names = [k.split("/", 1)[1] for k in want.files if k.startswith("%d/" % fmt)]
+    assert names
+    for n in names:
+        got = (tmp_path / n).read_bytes()
+        ref = bytes(want["%d/%s" % (fmt, n)])
+        if got != ref:
+            gl, rl = got.decode().splitlines(), ref.decode().splitlines()
+            bad = next((q for q, (a, b) in enumerate(zip(gl, rl)) if a != b), min(len(gl), len(rl)))
+            pytest.fail("%s differs from the reference's at line %d: %r vs %r" % (
+                n, bad + 1, gl[bad] if bad < len(gl) else None, rl[bad] if bad < len(rl) else None))
