@@ -414,21 +414,23 @@ template <typename T>
 __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
                          const int32_t* wall_cell, const int32_t* wall_corner,
                          const uint64_t* views, Dims d) {
-  const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (idx >= nwalls) return;
   f += blockIdx.y * d.rs_f;
   fw += blockIdx.y * d.rs_fw;
   const int lane = threadIdx.x & (kWave - 1);
-  const int w = walls[idx];
-  const int64_t cell = wall_cell[w];
-  const uint64_t vv = views[w];
-  for (int o = 0; o < 8; ++o) {
-    const int src = (int)((vv >> (8 * o + 4)) & 15u);
-    if (src == 0) continue;
-    const T* s = src <= 8 ? f + fat(d, src - 1, cell)
-                          : fw + ((size_t)wall_corner[w] * 8 + o) * d.Nc8;
-    T* dst = f + fat(d, o, cell);
-    for (int l = lane; l < d.Nc8; l += kWave) dst[l] = s[l];
+  const int wpb = blockDim.x / kWave;
+  // one wave per wall, striding over the list (the launch grid is capped)
+  for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < nwalls; idx += gridDim.x * wpb) {
+    const int w = walls[idx];
+    const int64_t cell = wall_cell[w];
+    const uint64_t vv = views[w];
+    for (int o = 0; o < 8; ++o) {
+      const int src = (int)((vv >> (8 * o + 4)) & 15u);
+      if (src == 0) continue;
+      const T* s = src <= 8 ? f + fat(d, src - 1, cell)
+                            : fw + ((size_t)wall_corner[w] * 8 + o) * d.Nc8;
+      T* dst = f + fat(d, o, cell);
+      for (int l = lane; l < d.Nc8; l += kWave) dst[l] = s[l];
+    }
   }
 }
 
@@ -598,12 +600,13 @@ __device__ void wall_dispatch(const WallArgs<T>& a, int idx, int lane) {
 // walls that read no other wall: all at once, one wave each
 template <typename T>
 __global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
-  const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (idx >= a.n) return;
   a.f += blockIdx.y * a.d.rs_f;
   a.fw += blockIdx.y * a.d.rs_fw;
   a.snap += blockIdx.y * a.d.rs_snap;
-  wall_dispatch<T, false>(a, idx, threadIdx.x & (kWave - 1));
+  const int wpb = blockDim.x / kWave;
+  // one wave per wall, striding over the list (the launch grid is capped)
+  for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < a.n; idx += gridDim.x * wpb)
+    wall_dispatch<T, false>(a, idx, threadIdx.x & (kWave - 1));
 }
 
 // walls that read another wall, one dependency level per launch: reads of
@@ -611,12 +614,12 @@ __global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
 // walls come from the snapshot taken before the wall pass
 template <typename T>
 __global__ __launch_bounds__(256) void k_walls_coupled(WallArgs<T> a) {
-  const int idx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (idx >= a.n) return;
   a.f += blockIdx.y * a.d.rs_f;
   a.fw += blockIdx.y * a.d.rs_fw;
   a.snap += blockIdx.y * a.d.rs_snap;
-  wall_dispatch<T, true>(a, idx, threadIdx.x & (kWave - 1));
+  const int wpb = blockDim.x / kWave;
+  for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < a.n; idx += gridDim.x * wpb)
+    wall_dispatch<T, true>(a, idx, threadIdx.x & (kWave - 1));
 }
 
 template <typename T>
@@ -757,21 +760,22 @@ __global__ void k_moments_serial(const T* f, const int32_t* cells, int64_t n, co
                                  const double* cy, const double* cz, Dims d, Mom* mom) {
   f += blockIdx.y * d.rs_f;
   mom += blockIdx.y * d.Ntotal;
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int64_t c = cells[t];
-  size_t b8[8];
-  for (int o = 0; o < 8; ++o) b8[o] = fat(d, o, c);
-  double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
-  for (int l = 0; l < d.Nc8; ++l) {
-    const double f1 = f[b8[0] + l], f2 = f[b8[1] + l], f3 = f[b8[2] + l], f4 = f[b8[3] + l];
-    const double f5 = f[b8[4] + l], f6 = f[b8[5] + l], f7 = f[b8[6] + l], f8 = f[b8[7] + l];
-    r = r + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
-    x = x + cx[l] * (f1 - f2 - f3 + f4 + f5 - f6 - f7 + f8);
-    y = y + cy[l] * (f1 + f2 - f3 - f4 + f5 + f6 - f7 - f8);
-    z = z + cz[l] * (f1 + f2 + f3 + f4 - f5 - f6 - f7 - f8);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = cells[t];
+    size_t b8[8];
+    for (int o = 0; o < 8; ++o) b8[o] = fat(d, o, c);
+    double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    for (int l = 0; l < d.Nc8; ++l) {
+      const double f1 = f[b8[0] + l], f2 = f[b8[1] + l], f3 = f[b8[2] + l], f4 = f[b8[3] + l];
+      const double f5 = f[b8[4] + l], f6 = f[b8[5] + l], f7 = f[b8[6] + l], f8 = f[b8[7] + l];
+      r = r + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
+      x = x + cx[l] * (f1 - f2 - f3 + f4 + f5 - f6 - f7 + f8);
+      y = y + cy[l] * (f1 + f2 - f3 - f4 + f5 + f6 - f7 - f8);
+      z = z + cz[l] * (f1 + f2 + f3 + f4 - f5 - f6 - f7 - f8);
+    }
+    mom[c] = Mom{r, x, y, z};
   }
-  mom[c] = Mom{r, x, y, z};
 }
 
 __global__ void k_gather_plane(const Mom* mom, double* out, Dims d, int column) {
@@ -817,6 +821,8 @@ int upload(Ctx& c, T** p, const std::vector<T>& v) {
   return 0;
 }
 
+// Blocks for n work items of `block` per block, at most 65536: every kernel
+// launched with it strides over its items.
 int grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));

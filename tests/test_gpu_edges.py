@@ -140,3 +140,44 @@ def test_refined_precision_on_a_multi_pass_grid():
         b.iterate(2)
         ka, kb = a.chk_converge()[1], b.chk_converge()[1]
         assert abs(ka - kb) <= 1e-5 * abs(ka)
+
+
+def _cube_lattice(n, period=5, size=3, off=2):
+    img = np.zeros((n, n, n), dtype=np.uint8)
+    for a in range(off, n - size, period):
+        for b in range(off, n - size, period):
+            for c in range(off, n - size, period):
+                img[a:a + size, b:b + size, c:c + size] = 1
+    return img
+
+
+def test_more_walls_than_one_launch_grid():
+    """316 342 walls (faces, edges and 3-fold corners of a lattice of solid
+    cubes): more than the 65 536 blocks x 4 waves one capped wall launch
+    covers, so every wall kernel must stride over its list.  Bit for bit
+    against the oracle after two iterations."""
+    print("case cube lattice 120^3", flush=True)
+    p = _params(120, 120, 120, 2, steps=2)
+    img = _cube_lattice(120)
+    from piks3d_amd import Solver
+    with Solver(p, img, precision="fp64") as s:
+        assert s.info.nwall > 65536 * 4
+    _compare(p, img, True, 2)
+
+
+def test_exact_order_moments_beyond_one_launch_grid():
+    """A fully fluid 168^3 image has 4.9 M fluid cells, more than the 65 536
+    blocks x 64 threads of one capped launch of the serial-order moments
+    kernel: exact and fast modes must agree on every cell's moments."""
+    print("case fluid 168^3 exact vs fast", flush=True)
+    from piks3d_amd import Solver
+    p = _params(168, 168, 168, 2, steps=2)
+    img = np.zeros((168, 168, 168), dtype=np.uint8)
+    out = []
+    for exact in (True, False):
+        with Solver(p, img, precision="fp64", exact_order=exact) as s:
+            assert s.info.nfluid > 65536 * 64
+            s.iterate(2)
+            out.append(np.stack(s.macro()))
+    scale = np.abs(out[1]).max()
+    assert np.abs(out[0] - out[1]).max() <= 1e-12 * scale
