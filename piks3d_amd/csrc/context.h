@@ -104,6 +104,7 @@ struct Ctx {
   double *e1 = nullptr, *e2 = nullptr;
   int32_t *uncoupled = nullptr, *coupled = nullptr, *coupled_snap = nullptr, *snap_cell = nullptr;
   int32_t* commit_walls = nullptr;
+  int32_t* wall_pos = nullptr;   // Geometry::wall_pos
   int32_t *fluid_int = nullptr, *fluid_bnd = nullptr, *fluid_all = nullptr;
   // sweep schedule: tasks of each oriented hyperplane
   bool xcd_remap = true;       // contiguous task ranges per XCD

@@ -357,6 +357,18 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
       g.uncoupled.push_back((int32_t)w);
     }
   }
+  // the uncoupled walls are independent of each other: group them by face
+  // count so that each group runs in a kernel sized for its register need
+  {
+    std::vector<int32_t> by_nf[3];
+    for (int32_t w : g.uncoupled) by_nf[wall_prog(g.wall_label[(size_t)w]).nface - 1].push_back(w);
+    g.uncoupled.clear();
+    for (int q = 0; q < 3; ++q) {
+      g.uncoupled_off[q] = (int32_t)g.uncoupled.size();
+      g.uncoupled.insert(g.uncoupled.end(), by_nf[q].begin(), by_nf[q].end());
+    }
+    g.uncoupled_off[3] = (int32_t)g.uncoupled.size();
+  }
   g.coupled_level_off.push_back(0);
   for (int L = 1; L <= maxlevel; ++L) {
     for (size_t w = 0; w < nwall; ++w)
@@ -479,6 +491,26 @@ void build_sweep_schedule(Geometry& g, int threads) {
       g.hp_len[(size_t)b] = hi - lo + 1;
     }
   }
+  });
+
+  // storage positions the wall kernels read and write
+  const size_t nwall = g.wall_cell.size();
+  const int K = Geometry::kWallPos;
+  g.wall_pos.assign(nwall * K, (int32_t)ns);
+  const int64_t stride[3] = {1, g.Nxt, g.Nxyt};
+  parallel_for((int)((nwall + 4095) / 4096), threads, [&](int chunk) {
+    for (size_t w = (size_t)chunk * 4096; w < std::min(nwall, (size_t)(chunk + 1) * 4096); ++w) {
+      const int64_t c = g.wall_cell[w];
+      int32_t* q = &g.wall_pos[w * K];
+      for (int F = 0; F < 4; ++F) q[F] = g.pos[F][(size_t)c];
+      const WallProg P = wall_prog(g.wall_label[w]);
+      for (int fa = 0; fa < P.nface; ++fa)
+        for (int d = 1; d <= 2; ++d)
+          for (int k = 0; k < 4; ++k) {
+            const int64_t n = c + (int64_t)d * P.face[fa].dir * stride[(int)P.face[fa].axis];
+            q[4 + fa * 8 + (d - 1) * 4 + k] = g.pos[ofam(P.face[fa].in[k])][(size_t)n];
+          }
+    }
   });
 }
 

@@ -63,7 +63,14 @@ struct Geometry {
   std::vector<int32_t> coupled_level_off; // offsets of each level in `coupled`
   std::vector<int32_t> coupled_snap;      // 6 per coupled wall: snapshot index or -1
   std::vector<int32_t> snap_cell;         // cells to snapshot before the wall pass
-  std::vector<int32_t> uncoupled;         // wall indices processed in parallel
+  std::vector<int32_t> uncoupled;         // wall indices processed in parallel, grouped
+                                          // by face count (1, 2, 3), raster order inside
+  int32_t uncoupled_off[4] = {0, 0, 0, 0};  // group of walls with n faces: [off[n-1], off[n])
+  // per wall, kWallPos storage positions: the wall cell in families 0..3,
+  // then per face (up to 3), per distance (1, 2), per incoming octant q (the
+  // face's in[q]) the neighbour cell in that octant's family
+  static constexpr int kWallPos = 4 + 3 * 2 * 4;
+  std::vector<int32_t> wall_pos;
   // fluid cells
   std::vector<int32_t> fluid_interior;    // not on any boundary plane: moments from sweep partials
   std::vector<int32_t> fluid_boundary;    // on i=1,Nx / j=1,Ny / k=1,Nz planes

@@ -412,7 +412,7 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
 // fired, else the y switch (src/solver.F90:184-204 and analogues).
 template <typename T>
 __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
-                         const int32_t* wall_cell, const int32_t* wall_corner,
+                         const int32_t* wall_pos, const int32_t* wall_corner,
                          const uint64_t* views, Dims d) {
   f += blockIdx.y * d.rs_f;
   fw += blockIdx.y * d.rs_fw;
@@ -421,14 +421,14 @@ __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
   // one wave per wall, striding over the list (the launch grid is capped)
   for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < nwalls; idx += gridDim.x * wpb) {
     const int w = walls[idx];
-    const int64_t cell = wall_cell[w];
+    const int32_t* wp = wall_pos + (size_t)w * Geometry::kWallPos;  // the wall cell per family
     const uint64_t vv = views[w];
     for (int o = 0; o < 8; ++o) {
       const int src = (int)((vv >> (8 * o + 4)) & 15u);
       if (src == 0) continue;
-      const T* s = src <= 8 ? f + fat(d, src - 1, cell)
+      const T* s = src <= 8 ? f + (size_t)(src - 1) * d.ostride + (size_t)wp[ofam(src - 1)] * d.Nc8
                             : fw + ((size_t)wall_corner[w] * 8 + o) * d.Nc8;
-      T* dst = f + fat(d, o, cell);
+      T* dst = f + (size_t)o * d.ostride + (size_t)wp[ofam(o)] * d.Nc8;
       for (int l = lane; l < d.Nc8; l += kWave) dst[l] = s[l];
     }
   }
@@ -478,6 +478,7 @@ struct WallArgs {
   const double* e1;
   const double* e2;
   const int32_t* wall_corner;
+  const int32_t* wall_pos;  // [wall][Geometry::kWallPos] storage positions
   const double* cx;
   const double* cy;
   const double* cz;
@@ -515,6 +516,54 @@ __device__ void wall_one(const WallArgs<T>& a, int idx, int lane) {
 
   // wall flux per face: R_a = sum_l c_a(l) (v0 + v1 + v2 + v3) / DiffFlux
   double R[3] = {0.0, 0.0, 0.0};
+  if (!a.exact && Nc8 <= kWave) {
+    // one velocity per lane: every incoming value is loaded once, the fluxes
+    // are summed from those values and the re-emission uses them; the same
+    // operations in the same order as the two passes below
+    const bool lact = lane < Nc8;
+    const int l = lact ? lane : 0;
+    // storage positions of the wall cell and of its neighbours (precomputed
+    // per wall; coupled walls read some neighbours from the snapshot instead)
+    const int32_t* wp = a.wall_pos + (size_t)wall * Geometry::kWallPos;
+    const size_t os = (size_t)a.d.ostride;
+    double v[24];
+#pragma unroll
+    for (int fa = 0; fa < P.nface; ++fa) {
+      const int64_t st = P.face[fa].dir * stride[P.face[fa].axis];
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int oo = P.face[fa].in[q];
+        double n1, n2;
+        if (COUPLED) {
+          n1 = load_nbr<T, COUPLED>(a, cell + st, oo, l, sidx[fa * 2]);
+          n2 = load_nbr<T, COUPLED>(a, cell + 2 * st, oo, l, sidx[fa * 2 + 1]);
+        } else {
+          n1 = (double)a.f[oo * os + (size_t)wp[4 + fa * 8 + q] * Nc8 + l];
+          n2 = (double)a.f[oo * os + (size_t)wp[4 + fa * 8 + 4 + q] * Nc8 + l];
+        }
+        const double val = e1 * n1 + e2 * n2;
+        v[P.face[fa].store + oo] = val;
+        s = q == 0 ? val : s + val;
+      }
+      R[fa] = lact ? 0.0 + cvel[P.face[fa].axis][l] * s : 0.0;
+    }
+#pragma unroll
+    for (int fa = 0; fa < P.nface; ++fa) R[fa] = wave_sum(R[fa]) / a.diff_flux;
+    const double aw = a.accom * a.w[l], b = 1.0 - a.accom;
+#pragma unroll
+    for (int t = 0; t < P.nwr; ++t) v[P.wr[t].dst] = aw * R[P.wr[t].rho] + b * v[P.wr[t].src];
+    if (!lact) return;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      if (!((written >> s) & 1u)) continue;
+      if (s < 8)
+        a.f[s * os + (size_t)wp[ofam(s)] * Nc8 + l] = (T)v[s];
+      else
+        a.fw[((size_t)corner * 8 + (s - 8)) * Nc8 + l] = (T)v[s];
+    }
+    return;
+  }
   if (!a.exact) {
     for (int l = lane; l < Nc8; l += kWave) {
 #pragma unroll
@@ -582,10 +631,15 @@ __device__ void wall_one(const WallArgs<T>& a, int idx, int lane) {
   }
 }
 
-template <typename T, bool COUPLED>
+// NF > 0: only the labels with NF faces are compiled in (a kernel for one
+// face-count group of the uncoupled walls needs fewer registers)
+template <typename T, bool COUPLED, int NF = 0>
 __device__ void wall_dispatch(const WallArgs<T>& a, int idx, int lane) {
   switch (a.wall_label[a.list[idx]]) {
-#define PK_CASE(L) case L: wall_one<L, T, COUPLED>(a, idx, lane); break;
+#define PK_CASE(L) \
+  case L:          \
+    if constexpr (NF == 0 || wall_prog(L).nface == NF) wall_one<L, T, COUPLED>(a, idx, lane); \
+    break;
     PK_CASE(L_E) PK_CASE(L_W) PK_CASE(L_N) PK_CASE(L_S) PK_CASE(L_F) PK_CASE(L_B)
     PK_CASE(L_EN) PK_CASE(L_WN) PK_CASE(L_ES) PK_CASE(L_WS)
     PK_CASE(L_NF) PK_CASE(L_NB) PK_CASE(L_SF) PK_CASE(L_SB)
@@ -597,8 +651,9 @@ __device__ void wall_dispatch(const WallArgs<T>& a, int idx, int lane) {
   }
 }
 
-// walls that read no other wall: all at once, one wave each
-template <typename T>
+// walls that read no other wall, all of one face count (NF): all at once,
+// one wave each
+template <typename T, int NF>
 __global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
   a.f += blockIdx.y * a.d.rs_f;
   a.fw += blockIdx.y * a.d.rs_fw;
@@ -606,7 +661,7 @@ __global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
   const int wpb = blockDim.x / kWave;
   // one wave per wall, striding over the list (the launch grid is capped)
   for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < a.n; idx += gridDim.x * wpb)
-    wall_dispatch<T, false>(a, idx, threadIdx.x & (kWave - 1));
+    wall_dispatch<T, false, NF>(a, idx, threadIdx.x & (kWave - 1));
 }
 
 // walls that read another wall, one dependency level per launch: reads of
@@ -859,6 +914,7 @@ WallArgs<T> wall_args(Ctx& c, State<T>& st, const int32_t* list, int n) {
   a.f = st.f; a.fw = st.fw; a.list = list; a.n = n;
   a.wall_cell = c.wall_cell; a.wall_label = c.wall_label; a.e1 = c.e1; a.e2 = c.e2;
   a.wall_corner = c.wall_corner;
+  a.wall_pos = c.wall_pos;
   a.cx = c.dcx; a.cy = c.dcy; a.cz = c.dcz; a.w = c.dw;
   a.accom = c.prm.accom; a.diff_flux = c.diff_flux; a.inv_diff_flux = 1.0 / c.diff_flux;
   a.d = c.d; a.exact = c.exact;
@@ -943,17 +999,19 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
     if (!g.commit_walls.empty())
       hipLaunchKernelGGL(k_commit<T>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb), R),
                          dim3(wpb * kWave), 0, c.stream, st.f, (const T*)st.fw, c.commit_walls,
-                         (int)g.commit_walls.size(), c.wall_cell, c.wall_corner, c.views, d);
+                         (int)g.commit_walls.size(), c.wall_pos, c.wall_corner, c.views, d);
     hipLaunchKernelGGL(k_halo<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256), R), dim3(256),
                        0, c.stream, st.f, st.buf_e, st.buf_w, d);
     if (!g.snap_cell.empty())
       hipLaunchKernelGGL(k_snapshot<T>, dim3(grid_for((int64_t)g.snap_cell.size() * 8 * d.Nc8, 256), R),
                          dim3(256), 0, c.stream, (const T*)st.f, st.snap, c.snap_cell,
                          (int)g.snap_cell.size(), d);
-    if (!g.uncoupled.empty()) {
-      WallArgs<T> wa = wall_args<T>(c, st, c.uncoupled, (int)g.uncoupled.size());
-      hipLaunchKernelGGL(k_walls<T>, dim3(grid_for((int64_t)g.uncoupled.size(), wpb), R),
-                         dim3(wpb * kWave), 0, c.stream, wa);
+    for (int nf = 1; nf <= 3; ++nf) {
+      const int b = g.uncoupled_off[nf - 1], n = g.uncoupled_off[nf] - b;
+      if (n == 0) continue;
+      WallArgs<T> wa = wall_args<T>(c, st, c.uncoupled + b, n);
+      void (*kw)(WallArgs<T>) = nf == 1 ? k_walls<T, 1> : nf == 2 ? k_walls<T, 2> : k_walls<T, 3>;
+      hipLaunchKernelGGL(kw, dim3(grid_for(n, wpb), R), dim3(wpb * kWave), 0, c.stream, wa);
     }
     for (size_t lv = 0; lv + 1 < g.coupled_level_off.size(); ++lv) {
       const int b = g.coupled_level_off[lv], n = g.coupled_level_off[lv + 1] - b;
@@ -1090,7 +1148,8 @@ size_t ctx_device_bytes(const Ctx& c) {
   const size_t state64 = R * (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(double);
   const size_t state32 = refined ? R * (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(float) : 0;
   const size_t per_cell = (size_t)g.Ntotal * (R * (sizeof(Mom) + 32 * sizeof(double)) + 1 + 4 + 4 * 4);
-  const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.uncoupled.size() * 4 +
+  const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.wall_pos.size() * 4 +
+                       g.uncoupled.size() * 4 +
                        g.coupled.size() * 4 * 7 + g.snap_cell.size() * 4 + g.commit_walls.size() * 4;
   const size_t fluid = (size_t)(g.fluid_interior.size() + g.fluid_boundary.size()) * 8;
   return state64 + state32 + per_cell + walls + fluid + g.hp_task.size() * 4 + g.hp_lo.size() * 8 +
@@ -1162,7 +1221,7 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
       (rc = upload(c, &c.wid, g.wid)) || (rc = upload(c, &c.views, g.wall_views)) ||
       (rc = upload(c, &c.wall_cell, g.wall_cell)) || (rc = upload(c, &c.wall_corner, g.wall_corner)) ||
       (rc = upload(c, &c.wall_label, g.wall_label)) || (rc = upload(c, &c.e1, g.wall_e1)) ||
-      (rc = upload(c, &c.e2, g.wall_e2)) || (rc = upload(c, &c.uncoupled, g.uncoupled)) ||
+      (rc = upload(c, &c.e2, g.wall_e2)) || (rc = upload(c, &c.uncoupled, g.uncoupled)) || (rc = upload(c, &c.wall_pos, g.wall_pos)) ||
       (rc = upload(c, &c.coupled, g.coupled)) || (rc = upload(c, &c.coupled_snap, g.coupled_snap)) ||
       (rc = upload(c, &c.snap_cell, g.snap_cell)) || (rc = upload(c, &c.commit_walls, g.commit_walls)) ||
       (rc = upload(c, &c.fluid_int, g.fluid_interior)) || (rc = upload(c, &c.fluid_bnd, g.fluid_boundary)))
