@@ -306,6 +306,42 @@ def c1_bench(args):
                    "eps": p.eps, "chk_converge_step": p.chk_converge_step}}))
 
 
+def c2_solve_bench(args):
+    """The 200^3 pack solved to the reference's criterion (check every 10
+    iterations, eps = --eps, at most --max-step iterations) in plain FP64 and
+    as the refined solve (FP32 iterations, FP64 finish): iterations, time to
+    solution, whole-solve updates/s and the permeability difference.  With
+    wallExtOrder 2 the reference's own iteration grows without bound on this
+    pack after about a hundred iterations, so solves use --wall-order 1."""
+    from piks3d_amd import Solver
+    p, img = workload()
+    p.chk_converge_step = 10
+    p.wall_ext_order = args.wall_order
+    rows = {}
+    for prec in ("fp64", "fp32"):
+        with Solver(p, img, precision=prec) as s:
+            nf, nc = s.info.nfluid, s.info.nc8 * 8
+            t = time.perf_counter()
+
+            def progress(step, m, k, e):
+                if step % 500 == 0:
+                    log("C2 %s: step %d, K %.12e, error %.3e" % (prec, step, k, e))
+            r = s.solve(max_step=args.max_step, eps=args.eps, callback=progress)
+            dt = time.perf_counter() - t
+        rows[prec] = {"steps": r["steps"], "steps_fp32": r.get("steps32", 0), "permeability": r["permeability"],
+                      "error": r["error"], "solve_s": dt, "updates_per_s": nf * nc * r["steps"] / dt}
+        log("C2 %s: %d steps (%d FP32), K %.12e, %.2f s" % (prec, r["steps"], r.get("steps32", 0),
+                                                          r["permeability"], dt))
+    a, b = rows["fp64"], rows["fp32"]
+    print(json.dumps({
+        "metric": "time to solution, 200^3 sphere pack, plain FP64 vs refined", "value": b["solve_s"],
+        "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "rows": rows,
+        "speedup_refined_vs_fp64": a["solve_s"] / b["solve_s"],
+        "rel_diff_permeability": abs(a["permeability"] - b["permeability"]) / abs(a["permeability"]),
+        "config": {"workload": "spheres200_phi040_kn1_v8", "eps": args.eps, "max_step": args.max_step,
+                   "chk_converge_step": 10, "wall_ext_order": args.wall_order}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +357,10 @@ def main():
                     help="run the serving stream instead (Poisson arrivals into the request server)")
     ap.add_argument("--kn-sweep", action="store_true",
                     help="run the multi-RHS Knudsen sweep on a 160^3 random field instead")
+    ap.add_argument("--c2-solve", action="store_true",
+                    help="the 200^3 pack to convergence, plain FP64 and refined")
+    ap.add_argument("--wall-order", type=int, default=2,
+                    help="--c2-solve: wallExtOrder (the reference's iteration diverges on this pack with 2)")
     ap.add_argument("--c1", action="store_true",
                     help="the 64^3 sphere configuration to convergence, GPU and reference program")
     ap.add_argument("--large", action="store_true",
@@ -360,6 +400,9 @@ def main():
         return
     if args.c1:
         c1_bench(args)
+        return
+    if args.c2_solve:
+        c2_solve_bench(args)
         return
 
     try:
