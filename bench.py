@@ -20,6 +20,7 @@ oracle/_ref), 2 iterations of the same workload on the host cores.
     python bench.py [--steps K] [--warmup W] [--precision fp64]
 """
 import argparse
+import hashlib
 import json
 import os
 import re
@@ -459,6 +460,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": wl, "image": "200^3 overlapping spheres r~U[4,10], porosity 0.40, seed %d" % SEED,
+                   "image_sha256": hashlib.sha256(img.tobytes()).hexdigest(),
                    "domain": [info.nx, info.ny, info.nz], "fluid_cells": info.nfluid,
                    "velocities": nc, "kn": p.kn, "precision": args.precision},
         "roofline": roof,

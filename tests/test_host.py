@@ -122,3 +122,19 @@ def test_image_generators_are_seeded_and_hit_their_porosity():
     assert abs(images.porosity(g) - 0.25) < 1e-3
     s = images.centered_sphere(64, 28.0)
     assert int(s.sum()) == 92096  # solid voxels of the 64^3 configuration
+
+
+def test_workload_images_match_their_committed_hashes():
+    """Every benchmark image regenerates to the SHA-256 committed with the
+    fixtures (tests/golden/make_workload_hashes.py wrote them)."""
+    import hashlib
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+    from make_workload_hashes import workload_images
+    want = {}
+    with open(os.path.join(REPO, "tests", "golden", "workload_images.sha256")) as fh:
+        for line in fh:
+            h, name = line.split()
+            want[name] = h
+    got = {name: hashlib.sha256(img.tobytes()).hexdigest() for name, img in workload_images()}
+    assert got == want
