@@ -1268,12 +1268,12 @@ static int capture_graph(Ctx& c) {
   const int rc = k ? enqueue_t<float>(c, c.s32, 1, false) : enqueue_t<double>(c, c.s64, 1, false);
   const hipError_t e = hipStreamEndCapture(c.stream, &gr);
   if (rc) {
-    if (gr) hipGraphDestroy(gr);
+    if (gr) (void)hipGraphDestroy(gr);
     return rc;
   }
   if (e != hipSuccess) return fail(PIKS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   const hipError_t ei = hipGraphInstantiate(&c.graph[k], gr, nullptr, nullptr, 0);
-  hipGraphDestroy(gr);
+  (void)hipGraphDestroy(gr);
   if (ei != hipSuccess) return fail(PIKS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   return 0;
 }
@@ -1369,17 +1369,19 @@ int ctx_set_phase(Ctx& c, int phase) {
 
 void ctx_destroy(Ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
+  // teardown is best effort: every release is attempted whatever the others
+  // return
+  (void)hipSetDevice(c->device);
   for (auto& g : c->graph)
-    if (g) hipGraphExecDestroy(g);
+    if (g) (void)hipGraphExecDestroy(g);
   if (c->stream_ordered) {
-    for (void* p : c->allocs) hipFreeAsync(p, c->stream);
+    for (void* p : c->allocs) (void)hipFreeAsync(p, c->stream);
   } else {
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void* p : c->allocs) hipFree(p);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
   }
-  for (auto& e : c->ev) hipEventDestroy(e);
-  if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+  for (auto& e : c->ev) (void)hipEventDestroy(e);
+  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 

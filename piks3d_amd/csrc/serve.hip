@@ -228,7 +228,7 @@ void scheduler(Server* s) {
       Slot& sl = *free_slot;
       const size_t need = (size_t)r->ctx->g.Ny * r->ctx->g.Nz;
       if (need > sl.plane_cap) {
-        if (sl.plane) hipHostFree(sl.plane);
+        if (sl.plane) (void)hipHostFree(sl.plane);
         sl.plane = nullptr;
         if (hipHostMalloc((void**)&sl.plane, need * sizeof(double), hipHostMallocDefault) != hipSuccess) {
           sl.plane_cap = 0;
@@ -394,10 +394,12 @@ int piks_serve_destroy(void* server) {
   s->cv_sched.notify_all();
   for (auto& t : s->workers) t.join();
   if (s->sched.joinable()) s->sched.join();
-  hipSetDevice(s->prm.device);
+  // teardown is best effort: every release is attempted whatever the others
+  // return
+  (void)hipSetDevice(s->prm.device);
   for (auto& sl : s->slots) {
     if (sl.req && sl.req->ctx) {
-      hipStreamSynchronize(sl.stream);
+      (void)hipStreamSynchronize(sl.stream);
       ctx_destroy(sl.req->ctx);
       sl.req->ctx = nullptr;
     }
@@ -406,11 +408,11 @@ int piks_serve_destroy(void* server) {
     if (kv.second->ctx) ctx_destroy(kv.second->ctx);
   for (auto& sl : s->slots) {
     if (sl.stream) {
-      hipStreamSynchronize(sl.stream);
-      hipStreamDestroy(sl.stream);
+      (void)hipStreamSynchronize(sl.stream);
+      (void)hipStreamDestroy(sl.stream);
     }
-    if (sl.done) hipEventDestroy(sl.done);
-    if (sl.plane) hipHostFree(sl.plane);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.plane) (void)hipHostFree(sl.plane);
   }
   delete s;
   return 0;
