@@ -333,8 +333,17 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
 // applied to the moments (an exact negation) instead of branching.  The
 // per-wave instruction count bounds this kernel, so every scalar operation on
 // its path counts.
+//
+// Occupancy hint per precision: the FP32 kernel scheduled for exactly 8 waves
+// per SIMD runs 4 % faster than the default schedule (same registers, 61 VGPRs);
+// the FP64 kernel keeps the default schedule (1..8 gives the same code), as
+// forcing it to 8 waves spills.
+template <typename T> struct SweepOcc { static constexpr int lo = 1, hi = 8; };
+template <> struct SweepOcc<float> { static constexpr int lo = 8, hi = 8; };
+
 template <typename T, bool kRecip>
-__global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SweepOcc<T>::lo, SweepOcc<T>::hi)))
+void k_sweep1(SweepArgs<T> a) {
   const uint32_t t = xcd_block(a.xcd_remap);
   if (t >= (uint32_t)a.ntask) return;
   const int lane = threadIdx.x;
