@@ -332,18 +332,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
 // wave, the flag-dependent coefficients read from a table, the octant's signs
 // applied to the moments (an exact negation) instead of branching.  The
 // per-wave instruction count bounds this kernel, so every scalar operation on
-// its path counts.
-//
-// Occupancy hint per precision: the FP32 kernel scheduled for exactly 8 waves
-// per SIMD runs 4 % faster than the default schedule (same registers, 61 VGPRs);
-// the FP64 kernel keeps the default schedule (1..8 gives the same code), as
-// forcing it to 8 waves spills.
-template <typename T> struct SweepOcc { static constexpr int lo = 1, hi = 8; };
-template <> struct SweepOcc<float> { static constexpr int lo = 8, hi = 8; };
-
+// its path counts.  Velocity grids of at most 64 velocities per octant only
+// (one pass of the wave; without the pass loop the FP64 kernel needs 52
+// VGPRs instead of 65, i.e. 8 waves per SIMD instead of 7); larger grids run
+// k_sweep, whose per-update arithmetic is the same.
 template <typename T, bool kRecip>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SweepOcc<T>::lo, SweepOcc<T>::hi)))
-void k_sweep1(SweepArgs<T> a) {
+__global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
   const uint32_t t = xcd_block(a.xcd_remap);
   if (t >= (uint32_t)a.ntask) return;
   const int lane = threadIdx.x;
@@ -362,8 +356,9 @@ void k_sweep1(SweepArgs<T> a) {
   const double mx = osx((int)o) > 0 ? m.x : -m.x;
   const double my = osy((int)o) > 0 ? m.y : -m.y;
   const double mz = osz((int)o) > 0 ? m.z : -m.z;
-  for (int base = 0; base < Nc8; base += kWave) {
-    const int l = base + lane;
+  {
+    // one pass: lane l holds velocity l of the octant (Nc8 <= 64)
+    const int l = lane;
     const bool lact = l < Nc8;
     const int ll = lact ? l : 0;
     const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
@@ -408,8 +403,7 @@ void k_sweep1(SweepArgs<T> a) {
         const int q = lane >> 4;
         const double zz = q == 1 ? (osx((int)o) > 0 ? z : -z) : q == 2 ? (osy((int)o) > 0 ? z : -z)
                         : q == 3 ? (osz((int)o) > 0 ? z : -z) : z;
-        double* pp = a.part + ((size_t)cell * 8 + o) * 4 + q;
-        *pp = base == 0 ? zz : *pp + zz;
+        a.part[((size_t)cell * 8 + o) * 4 + q] = zz;
       }
     }
   }
@@ -994,7 +988,7 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
       sa.win = c.hp_win + (size_t)h * 8;
       const int64_t wpg = c.sweep_block / kWave;
       const int64_t blocks = ((sa.ntask + wpg - 1) / wpg + 7) / 8 * 8;
-      if (c.nrhs == 1 && c.sweep_block == 64 && !c.sweep_generic) {
+      if (c.nrhs == 1 && c.sweep_block == 64 && !c.sweep_generic && d.Nc8 <= kWave) {
         const bool recip = c.sweep_recip && !c.exact;
         SweepKernel<T> k1 = recip ? k_sweep1<T, true> : k_sweep1<T, false>;
         hipLaunchKernelGGL(k1, dim3((unsigned)blocks), dim3(64), 0, c.stream, sa);
