@@ -61,10 +61,67 @@ constexpr int kMaxTimedSteps = 4096;  // steps of one piks_iterate call timed by
 
 // ------------------------------------------------------------------ helpers
 
+// Lane exchanges of 64-bit values in registers (no LDS round trip):
+// swap_halves(x, y) leaves x = [x lanes 0-31, y lanes 0-31] and y = [x lanes
+// 32-63, y lanes 32-63] (gfx950 v_permlane32_swap); swap_rows(x, y) leaves
+// x = [x row 0, y row 0, x row 2, y row 2] and y = [x row 1, y row 1, x row 3,
+// y row 3] for 16-lane rows (v_permlane16_swap); dpp<C>(v) reads v from the
+// lane that the DPP control C names inside the lane's row.
+__device__ __forceinline__ void swap_halves(double& x, double& y) {
+  const uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ux, (uint32_t)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+  x = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  y = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+}
+__device__ __forceinline__ void swap_rows(double& x, double& y) {
+  const uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)ux, (uint32_t)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(ux >> 32), (uint32_t)(uy >> 32), false, false);
+  x = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  y = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, kCtrl, 0xf, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), kCtrl, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+constexpr int kDppRowRor8 = 0x128, kDppRowRor4 = 0x124;   // rotate the row by 8, 4 lanes
+constexpr int kDppXor2 = 0x4e, kDppXor1 = 0xb1;           // quad_perm [2,3,0,1], [1,0,3,2]
+
+// Nibble o, bits 1-3: the octant's x, y, z signs are negative (bit 0, the
+// density, never is).
+constexpr uint32_t oct_neg_bits() {
+  uint32_t b = 0;
+  for (int o = 0; o < 8; ++o)
+    b |= (uint32_t)((osx(o) < 0 ? 2 : 0) | (osy(o) < 0 ? 4 : 0) | (osz(o) < 0 ? 8 : 0)) << (4 * o);
+  return b;
+}
+constexpr uint32_t kOctNeg = oct_neg_bits();
+static_assert(kOctNeg == 0xCEA84620u, "octant sign table");
+
+// Inside a 16-lane row whose values repeat with period 16 / 2^k, the row
+// rotations by 8 and 4 read the same value as the partner lane ^ 8 and ^ 4, so
+// the four steps below form the butterfly over lane ^ 8, ^ 4, ^ 2, ^ 1.
+__device__ __forceinline__ double row_sum(double z) {
+  z += dpp<kDppRowRor8>(z);
+  z += dpp<kDppRowRor4>(z);
+  z += dpp<kDppXor2>(z);
+  z += dpp<kDppXor1>(z);
+  return z;
+}
+
+// Sum over the wave, in every lane: the butterfly over lane ^ 32, ^ 16, ^ 8,
+// ^ 4, ^ 2, ^ 1 (each step adds the partner's value to the lane's own).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
-  return v;
+  double x = v, y = v;
+  swap_halves(x, y);
+  double z = x + y;
+  x = z; y = z;
+  swap_rows(x, y);
+  return row_sum(x + y);
 }
 
 __device__ __forceinline__ int64_t uniform64(int64_t v) {
@@ -145,22 +202,18 @@ __device__ __forceinline__ const T* switched_source(const ViewArgs<T>& a, const 
   return fw + ((size_t)a.wall_corner[w] * 8 + o) * a.d.Nc8;
 }
 
-// Sum of four per-lane values over the wave in 7 exchanges: lane 0 ends with
-// the sum of a, lane 16 of b, lane 32 of c, lane 48 of d.
-template <typename C>
-__device__ __forceinline__ C wave_sum4(C a, C b, C c, C d, int lane) {
-  const bool up = lane & 32;
-  const C r0 = __shfl_xor(up ? a : c, 32, kWave);
-  const C r1 = __shfl_xor(up ? b : d, 32, kWave);
-  const C x = (up ? c : a) + r0;
-  const C y = (up ? d : b) + r1;
-  const bool b4 = lane & 16;
-  C z = (b4 ? y : x) + __shfl_xor(b4 ? x : y, 16, kWave);
-  z += __shfl_xor(z, 8, kWave);
-  z += __shfl_xor(z, 4, kWave);
-  z += __shfl_xor(z, 2, kWave);
-  z += __shfl_xor(z, 1, kWave);
-  return z;
+// Sum of four per-lane values over the wave: lane 0 ends with the sum of a,
+// lane 16 of b, lane 32 of c, lane 48 of d.  The same butterfly as wave_sum
+// (lane ^ 32, then ^ 16, ^ 8, ^ 4, ^ 2, ^ 1), with the four sums sharing its
+// first two steps: after swap_halves the lower half adds a's halves and the
+// upper half c's (and b's, d's in the second pair), after swap_rows the rows
+// hold a, b, c, d partials.
+__device__ __forceinline__ double wave_sum4(double a, double b, double c, double d) {
+  swap_halves(a, c);
+  swap_halves(b, d);
+  double x = a + c, y = b + d;
+  swap_rows(x, y);
+  return row_sum(x + y);
 }
 
 // Workgroups are dispatched round-robin over the eight XCDs (block b runs on
@@ -315,7 +368,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
       if (a.part) {
         // moments are summed in FP64 in both phases
         const double fv = lact ? (double)(T)fnew : 0.0;
-        const double z = wave_sum4<double>(fv, cxl * fv, cyl * fv, czl * fv, lane);
+        const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv);
         if ((lane & 15) == 0) {
           const int q = lane >> 4;
           const double sgn = q == 1 ? sx : q == 2 ? sy : q == 3 ? sz : 1;
@@ -397,13 +450,12 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
     if (lact) buf_store((T)fnew, rs, voff, self);
     if (a.part) {
       const double fv = lact ? (double)(T)fnew : 0.0;
-      const double z = wave_sum4<double>(fv, cxl * fv, cyl * fv, czl * fv, lane);
+      const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv);
       if ((lane & 15) == 0) {
-        // the signs of the Ux, Uy, Uz partials are the octant's
-        const int q = lane >> 4;
-        const double zz = q == 1 ? (osx((int)o) > 0 ? z : -z) : q == 2 ? (osy((int)o) > 0 ? z : -z)
-                        : q == 3 ? (osz((int)o) > 0 ? z : -z) : z;
-        a.part[((size_t)cell * 8 + o) * 4 + q] = zz;
+        // the signs of the Ux, Uy, Uz partials (lanes 16, 32, 48) are the
+        // octant's: bit q of neg is set when partial q is negated
+        const uint32_t q = (uint32_t)lane >> 4;
+        a.part[((size_t)cell * 8 + o) * 4 + q] = ((kOctNeg >> (4 * o + q)) & 1u) ? -z : z;
       }
     }
   }
