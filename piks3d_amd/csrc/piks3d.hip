@@ -172,7 +172,7 @@ struct SweepArgs {
   int32_t nrhs;
   int32_t ntask;
   int32_t Nc8;
-  int32_t xcd_remap;
+  uint32_t xcd_per;        // blocks per XCD (grid / 8) when renumbering, else 0
 };
 
 // Everything a read of a switched wall slot needs (device memory).
@@ -220,11 +220,9 @@ __device__ __forceinline__ double wave_sum4(double a, double b, double c, double
 // XCD b % 8).  Renumber them so that XCD x runs the x-th contiguous eighth of
 // the launch's tasks: consecutive tasks of one octant share upwind
 // neighbours, which then stay in that XCD's L2.  Launch grids are multiples
-// of 8 blocks.
-__device__ __forceinline__ uint32_t xcd_block(bool remap) {
-  if (!remap) return blockIdx.x;
-  const uint32_t per = gridDim.x / 8;
-  return (blockIdx.x % 8) * per + blockIdx.x / 8;
+// of 8 blocks; per = grid / 8 comes with the arguments (0: no renumbering).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t per) {
+  return per ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
 }
 
 // Raw (untyped) buffer resources, word 3 for the gfx9 family: loads outside
@@ -295,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
   // the wave index is uniform over the wave: keep it (and everything derived
   // from it: the record, cell, octant, the uniform parts of addresses) in SGPRs
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint32_t t = xcd_block(a.xcd_remap) * (kThreads / kWave) + wave;
+  const uint32_t t = xcd_block(a.xcd_per) * (kThreads / kWave) + wave;
   if (t >= (uint32_t)a.ntask) return;
   const int lane = threadIdx.x & (kWave - 1);
   const TaskRec r = ld_const(a.tasks + t);
@@ -391,10 +389,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
 // k_sweep, whose per-update arithmetic is the same.
 template <typename T, bool kRecip>
 __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
-  const uint32_t t = xcd_block(a.xcd_remap);
-  if (t >= (uint32_t)a.ntask) return;
+  // the padding waves of the grid (t >= ntask) run the last task without
+  // storing anything, so that no load waits behind the bounds test
+  const uint32_t t = xcd_block(a.xcd_per);
+  const bool live = t < (uint32_t)a.ntask;
   const int lane = threadIdx.x;
-  const TaskRec r = ld_const(a.tasks + t);
+  const TaskRec r = ld_const(a.tasks + (live ? t : (uint32_t)a.ntask - 1u));
   const uint32_t o = (uint32_t)r.w[0] >> 28;
   const uint32_t self = ((uint32_t)r.w[0] & 0x0fffffffu) * (uint32_t)sizeof(T);
   const uint32_t cell = (uint32_t)r.w[7] & 0x0fffffffu;
@@ -447,8 +447,8 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
       den = den + czl * cf.dz;
       fnew = num / den;
     }
-    if (lact) buf_store((T)fnew, rs, voff, self);
-    if (a.part) {
+    if (lact && live) buf_store((T)fnew, rs, voff, self);
+    if (a.part && live) {
       const double fv = lact ? (double)(T)fnew : 0.0;
       const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv);
       if ((lane & 15) == 0) {
@@ -1017,7 +1017,6 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
   sa.c0 = 1.5 / g.ds; sa.c1 = 2.0 / g.ds; sa.c2 = -0.5 / g.ds; sa.d1 = 1.0 / g.ds;
   sa.view = static_cast<const ViewArgs<T>*>(c.view_args[sizeof(T) == sizeof(double) ? 0 : 1]);
   sa.Nc8 = d.Nc8;
-  sa.xcd_remap = c.xcd_remap;
   sa.nrhs = c.nrhs;
   sa.rs_f = d.rs_f;
   sa.rs_mom = d.Ntotal;
@@ -1040,6 +1039,7 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
       sa.win = c.hp_win + (size_t)h * 8;
       const int64_t wpg = c.sweep_block / kWave;
       const int64_t blocks = ((sa.ntask + wpg - 1) / wpg + 7) / 8 * 8;
+      sa.xcd_per = c.xcd_remap ? (uint32_t)(blocks / 8) : 0u;
       if (c.nrhs == 1 && c.sweep_block == 64 && !c.sweep_generic && d.Nc8 <= kWave) {
         const bool recip = c.sweep_recip && !c.exact;
         SweepKernel<T> k1 = recip ? k_sweep1<T, true> : k_sweep1<T, false>;
