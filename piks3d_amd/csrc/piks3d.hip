@@ -478,11 +478,31 @@ __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
     const int w = walls[idx];
     const int32_t* wp = wall_pos + (size_t)w * Geometry::kWallPos;  // the wall cell per family
     const uint64_t vv = views[w];
+    auto source = [&](int o, int src) -> const T* {
+      return src <= 8 ? f + (size_t)(src - 1) * d.ostride + (size_t)wp[ofam(src - 1)] * d.Nc8
+                      : fw + ((size_t)wall_corner[w] * 8 + o) * d.Nc8;
+    };
+    if (d.Nc8 <= kWave) {
+      // all eight reads in flight before the first write: no switch source is
+      // a switch target (checked statically in wallprog.h), so the reads see
+      // the sweep's values whatever the order
+      T v[8];
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int src = (int)((vv >> (8 * o + 4)) & 15u);
+        v[o] = (src != 0 && lane < d.Nc8) ? source(o, src)[lane] : T(0);
+      }
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int src = (int)((vv >> (8 * o + 4)) & 15u);
+        if (src != 0 && lane < d.Nc8) f[(size_t)o * d.ostride + (size_t)wp[ofam(o)] * d.Nc8 + lane] = v[o];
+      }
+      continue;
+    }
     for (int o = 0; o < 8; ++o) {
       const int src = (int)((vv >> (8 * o + 4)) & 15u);
       if (src == 0) continue;
-      const T* s = src <= 8 ? f + (size_t)(src - 1) * d.ostride + (size_t)wp[ofam(src - 1)] * d.Nc8
-                            : fw + ((size_t)wall_corner[w] * 8 + o) * d.Nc8;
+      const T* s = source(o, src);
       T* dst = f + (size_t)o * d.ostride + (size_t)wp[ofam(o)] * d.Nc8;
       for (int l = lane; l < d.Nc8; l += kWave) dst[l] = s[l];
     }
