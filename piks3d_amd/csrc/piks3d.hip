@@ -521,6 +521,27 @@ __global__ void k_halo(T* f, T* buf_e, T* buf_w, Dims d) {
   f += blockIdx.y * d.rs_f;
   buf_e += blockIdx.y * d.rs_buf;
   buf_w += blockIdx.y * d.rs_buf;
+  if (d.Nc8 <= kWave) {
+    // one wave per (row, octant), one velocity per lane: the row's storage
+    // positions are wave-uniform and all four reads are in flight together
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int items = d.Ny * d.Nz * 8;
+    for (int it = blockIdx.x * wpb + threadIdx.x / kWave; it < items; it += gridDim.x * wpb) {
+      const int o = it & 7, r = it >> 3;
+      const int j = 1 + r % d.Ny, k = 1 + r / d.Ny;
+      const int64_t row = (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+      if (lane >= d.Nc8) continue;
+      const size_t t = (size_t)it * d.Nc8 + lane;
+      const T ve = f[fat(d, o, row + d.Nx + 1) + lane], vw = f[fat(d, o, row + 2) + lane];
+      const T be = buf_e[t], bw = buf_w[t];
+      f[fat(d, o, row + 1) + lane] = be;
+      f[fat(d, o, row + d.Nx + 2) + lane] = bw;
+      buf_e[t] = ve;
+      buf_w[t] = vw;
+    }
+    return;
+  }
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -1075,8 +1096,10 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
       hipLaunchKernelGGL(k_commit<T>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb), R),
                          dim3(wpb * kWave), 0, c.stream, st.f, (const T*)st.fw, c.commit_walls,
                          (int)g.commit_walls.size(), c.wall_pos, c.wall_corner, c.views, d);
-    hipLaunchKernelGGL(k_halo<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256), R), dim3(256),
-                       0, c.stream, st.f, st.buf_e, st.buf_w, d);
+    hipLaunchKernelGGL(k_halo<T>,
+                       dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Ny * g.Nz * 8, wpb)
+                                           : grid_for((int64_t)g.Ny * g.Nz * 8 * d.Nc8, 256), R),
+                       dim3(256), 0, c.stream, st.f, st.buf_e, st.buf_w, d);
     if (!g.snap_cell.empty())
       hipLaunchKernelGGL(k_snapshot<T>, dim3(grid_for((int64_t)g.snap_cell.size() * 8 * d.Nc8, 256), R),
                          dim3(256), 0, c.stream, (const T*)st.f, st.snap, c.snap_cell,
