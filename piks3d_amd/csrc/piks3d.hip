@@ -419,6 +419,11 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
     double nb[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) nb[q] = buf_load<T>(rs, voff, (uint32_t)r.w[1 + q] * (uint32_t)sizeof(T));
+    // FP32: the velocity table is read before the switched-wall call (the
+    // compiler moves no load across it), so that these reads overlap the f
+    // reads (4 % faster); FP64 reads it after the call (1 % faster there)
+    double cxl, cyl, czl, wl;
+    if constexpr (sizeof(T) == 4) { cxl = a.cx[ll]; cyl = a.cy[ll]; czl = a.cz[ll]; wl = a.w[ll]; }
     const double kinv = kRecip ? a.inv_den[(flg & 7u) * (uint32_t)Nc8 + (uint32_t)ll] : 0.0;
     if (flg & 8u) {  // reads a wall slot the reference switched
       const ViewFix<T> vf = view_fix(a.view, cell, (int)o, ll, 0);
@@ -426,7 +431,7 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
       for (int q = 0; q < 4; ++q)
         if (vf.mask & (1u << q)) nb[2 + q] = vf.v[q];
     }
-    const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
+    if constexpr (sizeof(T) == 8) { cxl = a.cx[ll]; cyl = a.cy[ll]; czl = a.cz[ll]; wl = a.w[ll]; }
     double tt = cxl * mx;
     tt = tt + cyl * my;
     tt = tt + czl * mz;
