@@ -94,7 +94,7 @@ for _n in (2, 4, 6, 8, 12, 16):
 
 # long runs (history of the convergence check only)
 LONG_CASES = {
-    # 64^3 centred sphere r = 28, 8^3 velocities, Kn 0.1
+    # the CPU-runnable configuration: 64^3 centred sphere r = 28, 8^3 velocities, Kn 0.1
     "c1_sphere64": (lambda: images.centered_sphere(64, 28.0),
                     dict(fluid_layer=4, nc_fundamental=4, kn=0.1, press_drop=1.0, accom=1.0,
                          wall_ext_order=2, chk_converge_step=10, max_step=100000, eps=1e-6),
