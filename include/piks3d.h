@@ -3,7 +3,7 @@
  *
  * The reference (iPACT-Platform/PIKS3D) has no library interface: its hot path
  * is `subroutine iterate` (src/solver.F90:111) and `subroutine chkConverge`
- * (src/output.F90:60), called from the driver loop in src/main.F90:95-128 and
+ * (src/output.F90:60), called from the main loop in src/main.F90:95-128 and
  * operating on module globals set up by src/main.F90:75-89.  This header is the
  * boundary a host program (the Fortran host in piks3d_amd/fortran, the Python
  * package, or any ISO_C_BINDING / ctypes caller) binds instead.  Plain C types
@@ -55,7 +55,7 @@ typedef struct piks_params {
     double  kn;                    /* &flowNml Kn */
     double  press_drop;            /* &flowNml pressDrop */
     double  accom;                 /* &flowNml accom */
-    /* additions of this build (optional &gpuNml) */
+    /* additions of this library (optional &gpuNml) */
     int32_t precision;             /* PIKS_PREC_* */
     int32_t device;                /* HIP device ordinal */
     int32_t exact_quadrature;      /* 1: exact Gauss-Hermite rules instead of the

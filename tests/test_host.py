@@ -50,7 +50,7 @@ def test_preprocess_rejects_bad_bytes_and_x_boundary_corners():
         preprocess(p, np.zeros(10, np.uint8))
     assert e.value.code == -2
     # a block touching x = 1 with fluidLayer 0 makes 3-fold corners in the
-    # first two x layers: the reference aborts in MPI, this build refuses it
+    # first two x layers: the reference aborts in MPI, the library refuses it
     p0 = Params(nx=8, ny=8, nz=8, nc_fundamental=2, kn=1.0, press_drop=0.1, fluid_layer=0)
     img = np.zeros((8, 8, 8), np.uint8)
     img[3:6, 3:6, 1:4] = 1
