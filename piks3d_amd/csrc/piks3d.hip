@@ -383,12 +383,12 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
 // wave, the flag-dependent coefficients read from a table, the octant's signs
 // applied to the moments (an exact negation) instead of branching.  The
 // per-wave instruction count bounds this kernel, so every scalar operation on
-// its path counts.  Velocity grids of at most 64 velocities per octant only
-// (one pass of the wave; without the pass loop the FP64 kernel needs 52
-// VGPRs instead of 65, i.e. 8 waves per SIMD instead of 7); larger grids run
-// k_sweep, whose per-update arithmetic is the same.
-template <typename T, bool kRecip>
-__global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
+// its path counts.  k_sweep1 serves grids of at most 64 velocities per octant
+// in one pass of the wave (without the pass loop the FP64 kernel needs 52
+// VGPRs instead of 65: 8 waves per SIMD instead of 7); k_sweep1_passes loops
+// over passes of 64 velocities for larger grids.
+template <typename T, bool kRecip, bool kOnePass>
+__device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
   // the padding waves of the grid (t >= ntask) run the last task without
   // storing anything, so that no load waits behind the bounds test
   const uint32_t t = xcd_block(a.xcd_per);
@@ -409,9 +409,9 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
   const double mx = osx((int)o) > 0 ? m.x : -m.x;
   const double my = osy((int)o) > 0 ? m.y : -m.y;
   const double mz = osz((int)o) > 0 ? m.z : -m.z;
-  {
-    // one pass: lane l holds velocity l of the octant (Nc8 <= 64)
-    const int l = lane;
+  // lane l holds velocity base + l of the octant; one pass when Nc8 <= 64
+  for (int base = 0; base < (kOnePass ? 1 : Nc8); base += kWave) {
+    const int l = base + lane;
     const bool lact = l < Nc8;
     const int ll = lact ? l : 0;
     const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
@@ -460,11 +460,21 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) {
         // the signs of the Ux, Uy, Uz partials (lanes 16, 32, 48) are the
         // octant's: bit q of neg is set when partial q is negated
         const uint32_t q = (uint32_t)lane >> 4;
-        a.part[((size_t)cell * 8 + o) * 4 + q] = ((kOctNeg >> (4 * o + q)) & 1u) ? -z : z;
+        const double zz = ((kOctNeg >> (4 * o + q)) & 1u) ? -z : z;
+        double* pp = a.part + ((size_t)cell * 8 + o) * 4 + q;
+        // passes after the first add to the first pass's sum
+        *pp = base == 0 ? zz : *pp + zz;
       }
     }
   }
 }
+
+template <typename T, bool kRecip>
+__global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) { sweep1_task<T, kRecip, true>(a); }
+
+// the same task for grids of more than 64 velocities per octant: passes of 64
+template <typename T, bool kRecip>
+__global__ __launch_bounds__(64) void k_sweep1_passes(SweepArgs<T> a) { sweep1_task<T, kRecip, false>(a); }
 
 // ------------------------------------------------------------------ switch commit
 
@@ -1086,9 +1096,10 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
       const int64_t wpg = c.sweep_block / kWave;
       const int64_t blocks = ((sa.ntask + wpg - 1) / wpg + 7) / 8 * 8;
       sa.xcd_per = c.xcd_remap ? (uint32_t)(blocks / 8) : 0u;
-      if (c.nrhs == 1 && c.sweep_block == 64 && !c.sweep_generic && d.Nc8 <= kWave) {
+      if (c.nrhs == 1 && c.sweep_block == 64 && !c.sweep_generic) {
         const bool recip = c.sweep_recip && !c.exact;
-        SweepKernel<T> k1 = recip ? k_sweep1<T, true> : k_sweep1<T, false>;
+        SweepKernel<T> k1 = d.Nc8 <= kWave ? (recip ? k_sweep1<T, true> : k_sweep1<T, false>)
+                                           : (recip ? k_sweep1_passes<T, true> : k_sweep1_passes<T, false>);
         hipLaunchKernelGGL(k1, dim3((unsigned)blocks), dim3(64), 0, c.stream, sa);
         continue;
       }
