@@ -225,6 +225,12 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t per) {
   return per ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
 }
 
+// Index of the calling wave inside its workgroup, marked wave-uniform: every
+// per-item load and branch that follows then runs on the scalar unit.
+__device__ __forceinline__ int wave_in_block() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+}
+
 // Raw (untyped) buffer resources, word 3 for the gfx9 family: loads outside
 // the resource's byte range return 0 and stores outside it are dropped.
 constexpr int kBufferWord3 = 0x00020000;
@@ -489,7 +495,7 @@ __global__ void k_commit(T* f, const T* fw, const int32_t* walls, int nwalls,
   const int lane = threadIdx.x & (kWave - 1);
   const int wpb = blockDim.x / kWave;
   // one wave per wall, striding over the list (the launch grid is capped)
-  for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < nwalls; idx += gridDim.x * wpb) {
+  for (int idx = blockIdx.x * wpb + wave_in_block(); idx < nwalls; idx += gridDim.x * wpb) {
     const int w = walls[idx];
     const int32_t* wp = wall_pos + (size_t)w * Geometry::kWallPos;  // the wall cell per family
     const uint64_t vv = views[w];
@@ -542,7 +548,7 @@ __global__ void k_halo(T* f, T* buf_e, T* buf_w, Dims d) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wpb = blockDim.x / kWave;
     const int items = d.Ny * d.Nz * 8;
-    for (int it = blockIdx.x * wpb + threadIdx.x / kWave; it < items; it += gridDim.x * wpb) {
+    for (int it = blockIdx.x * wpb + wave_in_block(); it < items; it += gridDim.x * wpb) {
       const int o = it & 7, r = it >> 3;
       const int j = 1 + r % d.Ny, k = 1 + r / d.Ny;
       const int64_t row = (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
@@ -771,7 +777,7 @@ __global__ __launch_bounds__(256) void k_walls(WallArgs<T> a) {
   a.snap += blockIdx.y * a.d.rs_snap;
   const int wpb = blockDim.x / kWave;
   // one wave per wall, striding over the list (the launch grid is capped)
-  for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < a.n; idx += gridDim.x * wpb)
+  for (int idx = blockIdx.x * wpb + wave_in_block(); idx < a.n; idx += gridDim.x * wpb)
     wall_dispatch<T, false, NF>(a, idx, threadIdx.x & (kWave - 1));
 }
 
@@ -784,7 +790,7 @@ __global__ __launch_bounds__(256) void k_walls_coupled(WallArgs<T> a) {
   a.fw += blockIdx.y * a.d.rs_fw;
   a.snap += blockIdx.y * a.d.rs_snap;
   const int wpb = blockDim.x / kWave;
-  for (int idx = blockIdx.x * wpb + threadIdx.x / kWave; idx < a.n; idx += gridDim.x * wpb)
+  for (int idx = blockIdx.x * wpb + wave_in_block(); idx < a.n; idx += gridDim.x * wpb)
     wall_dispatch<T, true>(a, idx, threadIdx.x & (kWave - 1));
 }
 
@@ -900,7 +906,7 @@ __global__ __launch_bounds__(256) void k_moments_direct(const T* f, const int32_
                                                         const double* cz, Dims d, Mom* mom) {
   f += blockIdx.y * d.rs_f;
   mom += blockIdx.y * d.Ntotal;
-  const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave_in_block();
   if (idx >= n) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t c = cells[idx];
