@@ -816,6 +816,27 @@ template <typename T>
 __global__ void k_inout(T* f, const double* w, double pd, Dims d) {
   const int64_t n = (int64_t)d.Ny * d.Nz * 4 * d.Nc8;
   f += blockIdx.y * d.rs_f;
+  if (d.Nc8 <= kWave) {
+    // one wave per (row, octant pair), one velocity per lane; the two reads
+    // (different octants, so different slots) are issued before the writes
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int items = d.Ny * d.Nz * 4;
+    for (int it = blockIdx.x * wpb + wave_in_block(); it < items; it += gridDim.x * wpb) {
+      const int q = it & 3, r = it >> 2;
+      const int j = 1 + r % d.Ny, k = 1 + r / d.Ny;
+      const int64_t row = (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+      const int oin = q == 0 ? 0 : q == 1 ? 3 : q == 2 ? 4 : 7;
+      const int oout = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 5 : 6;
+      if (lane >= d.Nc8) continue;
+      const double wpd = w[lane] * pd;
+      const double a = (double)f[fat(d, oin, row + 1) + lane];
+      const double b = (double)f[fat(d, oout, row + d.Nx + 2) + lane];
+      f[fat(d, oin, row + 2) + lane] = (T)(a + wpd);
+      f[fat(d, oout, row + d.Nx + 1) + lane] = (T)(b - wpd);
+    }
+    return;
+  }
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -838,11 +859,40 @@ template <typename T>
 __device__ __forceinline__ void copy_slot(T* f, const Dims& d, int64_t cell, int l, int dst, int src) {
   f[fat(d, dst, cell) + l] = f[fat(d, src, cell) + l];
 }
+// four copies dst_k <- src_k in one cell, all reads before the writes (the
+// four sources and the four targets are eight different octants)
+template <typename T>
+__device__ __forceinline__ void mirror4(T* f, const Dims& d, int64_t cell, int l, int d0, int s0, int d1,
+                                        int s1, int d2, int s2, int d3, int s3) {
+  const T v0 = f[fat(d, s0, cell) + l], v1 = f[fat(d, s1, cell) + l];
+  const T v2 = f[fat(d, s2, cell) + l], v3 = f[fat(d, s3, cell) + l];
+  f[fat(d, d0, cell) + l] = v0;
+  f[fat(d, d1, cell) + l] = v1;
+  f[fat(d, d2, cell) + l] = v2;
+  f[fat(d, d3, cell) + l] = v3;
+}
 
 template <typename T>
 __global__ void k_sym_y(T* f, Dims d) {
   const int64_t n = (int64_t)d.Nx * d.Nz * d.Nc8;
   f += blockIdx.y * d.rs_f;
+  if (d.Nc8 <= kWave) {
+    // one wave per (i, k), one velocity per lane: the four reads of a plane's
+    // cell are issued before its four writes (sources and targets are
+    // different octants); the south plane is done before the north one
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int items = d.Nx * d.Nz;
+    for (int it = blockIdx.x * wpb + wave_in_block(); it < items; it += gridDim.x * wpb) {
+      const int i = 1 + it % d.Nx, k = 1 + it / d.Nx;
+      if (lane >= d.Nc8) continue;
+      const int64_t s = (int64_t)(i + 1) + 2 * (int64_t)d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+      mirror4<T>(f, d, s, lane, 1, 2, 0, 3, 5, 6, 4, 7);
+      const int64_t q = (int64_t)(i + 1) + (int64_t)(d.Ny + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+      mirror4<T>(f, d, q, lane, 2, 1, 3, 0, 6, 5, 7, 4);
+    }
+    return;
+  }
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -862,6 +912,21 @@ template <typename T>
 __global__ void k_sym_z(T* f, Dims d) {
   const int64_t n = (int64_t)d.Nx * d.Ny * d.Nc8;
   f += blockIdx.y * d.rs_f;
+  if (d.Nc8 <= kWave) {
+    // as k_sym_y: one wave per (i, j), back plane before front plane
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int items = d.Nx * d.Ny;
+    for (int it = blockIdx.x * wpb + wave_in_block(); it < items; it += gridDim.x * wpb) {
+      const int i = 1 + it % d.Nx, j = 1 + it / d.Nx;
+      if (lane >= d.Nc8) continue;
+      const int64_t s = (int64_t)(i + 1) + (int64_t)(j + 1) * d.Nxt + 2 * (int64_t)d.Nxyt;
+      mirror4<T>(f, d, s, lane, 0, 4, 1, 5, 2, 6, 3, 7);
+      const int64_t q = (int64_t)(i + 1) + (int64_t)(j + 1) * d.Nxt + (int64_t)(d.Nz + 1) * d.Nxyt;
+      mirror4<T>(f, d, q, lane, 4, 0, 5, 1, 6, 2, 7, 3);
+    }
+    return;
+  }
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % d.Nc8);
@@ -1140,11 +1205,11 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
       hipLaunchKernelGGL(k_walls_coupled<T>, dim3(grid_for(n, wpb), R), dim3(wpb * kWave), 0,
                          c.stream, wa);
     }
-    hipLaunchKernelGGL(k_inout<T>, dim3(grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256), R), dim3(256),
+    hipLaunchKernelGGL(k_inout<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Ny * g.Nz * 4, wpb) : grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256), R), dim3(256),
                        0, c.stream, st.f, (const double*)c.dw, c.prm.press_drop, d);
-    hipLaunchKernelGGL(k_sym_y<T>, dim3(grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256), R), dim3(256), 0,
+    hipLaunchKernelGGL(k_sym_y<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Nz, wpb) : grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256), R), dim3(256), 0,
                        c.stream, st.f, d);
-    hipLaunchKernelGGL(k_sym_z<T>, dim3(grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256), R), dim3(256), 0,
+    hipLaunchKernelGGL(k_sym_z<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Ny, wpb) : grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256), R), dim3(256), 0,
                        c.stream, st.f, d);
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 2], c.stream));
     if (c.exact) {
