@@ -254,7 +254,9 @@ __device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uin
 }
 
 // Values seen by a cell whose y or z upwind neighbours are switched wall
-// slots (out of line: few tasks take this path).
+// slots (few tasks take this path).  view_fix is the out-of-line call;
+// view_fix_body the same code for callers that inline it (the FP64
+// single-solve sweep: 4.6 % faster inlined, the FP32 one is not).
 template <typename T>
 struct ViewFix {
   T v[4];                  // y1 y2 z1 z2
@@ -262,7 +264,8 @@ struct ViewFix {
 };
 
 template <typename T>
-__device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell, int o, int ll, int rhs) {
+__device__ __forceinline__ ViewFix<T> view_fix_body(const ViewArgs<T>* va, uint32_t cell, int o, int ll,
+                                                    int rhs) {
   const ViewArgs<T>& a = *va;
   const T* f = a.f + rhs * a.d.rs_f;
   const T* fw = a.fw + rhs * a.d.rs_fw;
@@ -278,6 +281,11 @@ __device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell
     }
   }
   return r;
+}
+
+template <typename T>
+__device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell, int o, int ll, int rhs) {
+  return view_fix_body(va, cell, o, ll, rhs);
 }
 
 // One hyperplane of the eight octant sweeps.  Every task (cell, octant) has
@@ -432,7 +440,8 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
     if constexpr (sizeof(T) == 4) { cxl = a.cx[ll]; cyl = a.cy[ll]; czl = a.cz[ll]; wl = a.w[ll]; }
     const double kinv = kRecip ? a.inv_den[(flg & 7u) * (uint32_t)Nc8 + (uint32_t)ll] : 0.0;
     if (flg & 8u) {  // reads a wall slot the reference switched
-      const ViewFix<T> vf = view_fix(a.view, cell, (int)o, ll, 0);
+      const ViewFix<T> vf = sizeof(T) == 8 ? view_fix_body(a.view, cell, (int)o, ll, 0)
+                                           : view_fix(a.view, cell, (int)o, ll, 0);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if (vf.mask & (1u << q)) nb[2 + q] = vf.v[q];
