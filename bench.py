@@ -94,27 +94,41 @@ def traffic_from_profiles(workload_name):
 SERVE_SIDES = (32, 48, 64, 96, 128)
 
 
-def serve_workload(nreq, seed, max_step, eps):
-    """The serving stream: nreq seeded sphere packs, side uniform over
-    SERVE_SIDES, porosity U[0.3, 0.5], Kn log-uniform over [0.01, 10], velocity
-    grid 6^3 for Kn < 0.1, 8^3 for Kn < 1, else 12^3 (half-range), 4 fluid
-    layers, unit pressure drop."""
+def serve_specs(nreq, seed, max_step, eps):
+    """The serving stream's requests without their images: nreq seeded
+    sphere packs, side uniform over SERVE_SIDES, porosity U[0.3, 0.5], Kn
+    log-uniform over [0.01, 10], velocity grid 6^3 for Kn < 0.1, 8^3 for
+    Kn < 1, else 12^3 (half-range), 4 fluid layers, unit pressure drop.
+    Returns (Params, porosity, image seed) per request."""
     import numpy as np
-    from piks3d_amd import images, preprocess
     from piks3d_amd.params import Params
     rng = np.random.Generator(np.random.PCG64(seed))
-    reqs = []
+    specs = []
     for q in range(nreq):
         side = int(rng.choice(SERVE_SIDES))
         phi = float(rng.uniform(0.3, 0.5))
         kn = float(10.0 ** rng.uniform(-2.0, 1.0))
         nc = 3 if kn < 0.1 else (4 if kn < 1.0 else 6)
-        img = images.sphere_pack(side, phi, 4.0, 10.0, seed=seed * 100003 + q)
         p = Params(nx=side, ny=side, nz=side, fluid_layer=4, wall_ext_order=2, nc_fundamental=nc,
                    half_range=True, exact_quadrature=(nc == 3), chk_converge_step=10,
                    max_step=max_step, eps=eps, kn=kn, press_drop=1.0, accom=1.0)
+        specs.append((p, phi, seed * 100003 + q))
+    return specs
+
+
+def serve_image(p, phi, img_seed):
+    from piks3d_amd import images
+    return images.sphere_pack(p.nx, phi, 4.0, 10.0, seed=img_seed)
+
+
+def serve_workload(nreq, seed, max_step, eps):
+    """The serving stream (serve_specs) with images and update counts."""
+    from piks3d_amd import preprocess
+    reqs = []
+    for q, (p, phi, img_seed) in enumerate(serve_specs(nreq, seed, max_step, eps)):
+        img = serve_image(p, phi, img_seed)
         _, info = preprocess(p, img)
-        reqs.append((p, img, info.nfluid * 8 * nc ** 3))
+        reqs.append((p, img, info.nfluid * 8 * p.nc_fundamental ** 3))
         if (q + 1) % 16 == 0:
             log("generated %d of %d" % (q + 1, nreq))
     return reqs

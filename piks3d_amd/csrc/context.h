@@ -116,6 +116,13 @@ struct Ctx {
   SweepCoef* coef = nullptr;   // [first-order flags]
   int32_t* hp_task = nullptr;
   SweepWin* hp_win = nullptr;  // storage window per (hyperplane, octant)
+  // tiled sweep (k_tile): one launch per tile diagonal instead of one per
+  // hyperplane, for grids of at most 64 velocities per octant; selected with
+  // PIKS_SWEEP_TILES=1 (measured slower than the hyperplane sweep on the
+  // 200^3 sphere pack for every tile shape tried)
+  bool sweep_tiles = false;
+  int tile_tx = 2, tile_ty = 2, tile_tz = 2;
+  uint32_t* tl_rec = nullptr;
   void* view_args[2] = {nullptr, nullptr};  // per precision, device copies for the sweep
   int32_t* pos[4] = {nullptr, nullptr, nullptr, nullptr};
   // one iteration captured as a HIP graph, per precision (used by ctx_enqueue
@@ -137,7 +144,8 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
 size_t ctx_device_bytes(const Ctx& c);
 // Device phase: allocate, clear and upload.  stream == nullptr: the context
 // creates its own stream and uses hipMalloc; otherwise it works on `stream`
-// with stream-ordered allocations.  Does not synchronise when given a stream.
+// with stream-ordered allocations.  Returns after the uploads have completed
+// (it synchronises the stream it works on).
 int ctx_create_device(Ctx& c, hipStream_t stream);
 // Enqueue nsteps iterations (no synchronisation, no timing).
 int ctx_enqueue(Ctx& c, int nsteps);

@@ -386,6 +386,8 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
   return PIKS_OK;
 }
 
+void build_wall_positions(Geometry& g, int threads);
+
 void build_sweep_schedule(Geometry& g, int threads) {
   // cells that are stored
   std::vector<uint8_t> stored((size_t)g.Ntotal, 0);
@@ -493,6 +495,173 @@ void build_sweep_schedule(Geometry& g, int threads) {
   }
   });
 
+  build_wall_positions(g, threads);
+}
+
+bool build_tile_schedule(Geometry& g, int TX, int TY, int TZ, int threads) {
+  const int NS = TX * TY;
+  const int W = Geometry::kTileRecDw;
+  g.tl_rec.clear();
+  g.tl_off.clear();
+  g.tl_ntile = 0;
+  g.tl_ndiag = 0;
+  g.tl_maxwin = 0;
+  if (TX < 2 || TY < 2 || TZ < 2 || tile_rec_cell(NS, TX, TY, NS) > W || tile_rec_zhalo(NS, 2) > W ||
+      NS > 32)
+    return false;
+  g.tl_tx = TX; g.tl_ty = TY; g.tl_tz = TZ;
+  const int NTX = (g.Nx + TX - 1) / TX, NTY = (g.Ny + TY - 1) / TY, NTZ = (g.Nz + TZ - 1) / TZ;
+  const int ND = NTX + NTY + NTZ - 2;
+  const int64_t zero = g.nstore;
+  // physical cell of oriented (oi, oj, ok) in octant o; the oriented range
+  // [-2, N+1] covers the ghosted grid
+  auto phys = [&](int o, int oi, int oj, int ok) {
+    const int i = osx(o) > 0 ? oi + 1 : g.Nx - oi;
+    const int j = osy(o) > 0 ? oj + 1 : g.Ny - oj;
+    const int k = osz(o) > 0 ? ok + 1 : g.Nz - ok;
+    return g.cell(i, j, k);
+  };
+  auto inside = [&](int oi, int oj, int ok) {
+    return oi >= 0 && oi < g.Nx && oj >= 0 && oj < g.Ny && ok >= 0 && ok < g.Nz;
+  };
+  auto swept = [&](int o, int oi, int oj, int ok) {
+    return inside(oi, oj, ok) && oi >= 1 && oj >= 1 && ok >= 1 && g.lab[(size_t)phys(o, oi, oj, ok)] == FLUID;
+  };
+  // tiles with a swept cell, per octant
+  struct Tile { int32_t d, o, I, J, K; };
+  std::vector<std::vector<Tile>> per(8);
+  parallel_for(8, threads, [&](int o) {
+    for (int K = 0; K < NTZ; ++K)
+      for (int J = 0; J < NTY; ++J)
+        for (int I = 0; I < NTX; ++I) {
+          bool any = false;
+          for (int c = 0; c < TZ && !any; ++c)
+            for (int b = 0; b < TY && !any; ++b)
+              for (int a = 0; a < TX && !any; ++a)
+                any = swept(o, I * TX + a, J * TY + b, K * TZ + c);
+          if (any) per[(size_t)o].push_back(Tile{I + J + K, o, I, J, K});
+        }
+  });
+  std::vector<Tile> all;
+  for (auto& v : per) all.insert(all.end(), v.begin(), v.end());
+  // diagonal, then octant, then K, J, I (each octant's list is already in
+  // K, J, I order, so a stable sort on (diagonal, octant) suffices)
+  std::stable_sort(all.begin(), all.end(),
+                   [](const Tile& x, const Tile& y) { return x.d != y.d ? x.d < y.d : x.o < y.o; });
+  g.tl_ntile = (int64_t)all.size();
+  g.tl_ndiag = ND;
+  g.tl_off.assign((size_t)ND + 1, 0);
+  for (const Tile& t : all) ++g.tl_off[(size_t)t.d + 1];
+  for (int d = 0; d < ND; ++d) g.tl_off[(size_t)d + 1] += g.tl_off[(size_t)d];
+  const size_t per_tile = (size_t)(TZ + 1) * W;
+  g.tl_rec.assign(all.size() * per_tile, 0);
+  std::atomic<int64_t> gmax{0}, gspan{0};
+  const int kChunk = 256;
+  parallel_for((int)((all.size() + kChunk - 1) / kChunk), threads, [&](int chunk) {
+    int64_t local_max = 0, local_span = 0;
+    std::vector<int64_t> posv;
+    for (size_t t = (size_t)chunk * kChunk; t < std::min(all.size(), (size_t)(chunk + 1) * kChunk); ++t) {
+      const Tile& T = all[t];
+      const int o = T.o;
+      const std::vector<int32_t>& pos = g.pos[ofam(o)];
+      uint32_t* R = &g.tl_rec[t * per_tile];
+      const int oi0 = T.I * TX, oj0 = T.J * TY, ok0 = T.K * TZ;
+      const int np = std::min(TZ, g.Nz - ok0);
+      // storage positions (absolute; the zero slot is `zero`), gathered first
+      // to find the window
+      auto sp = [&](int oi, int oj, int ok, bool need_inside) -> int64_t {
+        if (need_inside && !inside(oi, oj, ok)) return zero;
+        if (oi < -2 || oj < -2 || ok < -2 || oi > g.Nx + 1 || oj > g.Ny + 1 || ok > g.Nz + 1) return zero;
+        return pos[(size_t)phys(o, oi, oj, ok)];
+      };
+      int64_t lo = INT64_MAX, hi = -1;
+      auto note = [&](int64_t p) {
+        if (p != zero) { lo = std::min(lo, p); hi = std::max(hi, p); }
+        return p;
+      };
+      // header
+      R[0] = (uint32_t)o | (uint32_t)np << 8;
+      posv.clear();
+      for (int w = 0; w < 2; ++w)
+        for (int b = 0; b < TY; ++b)
+          for (int a = 0; a < TX; ++a) {
+            // planes below the tile: only cells inside the domain in x, y
+            const int oi = oi0 + a, oj = oj0 + b, ok = ok0 - 2 + w;
+            const bool ok_xy = oi < g.Nx && oj < g.Ny;
+            posv.push_back(note(ok_xy ? sp(oi, oj, ok, false) : zero));
+          }
+      for (int c = 0; c < np; ++c) {
+        const int ok = ok0 + c;
+        for (int b = 0; b < TY; ++b)
+          for (int a = 0; a < TX; ++a) posv.push_back(note(sp(oi0 + a, oj0 + b, ok, true)));
+        for (int b = 0; b < TY; ++b)
+          for (int ap = -2; ap <= -1; ++ap) {
+            const int oj = oj0 + b;
+            posv.push_back(note(oj < g.Ny ? sp(oi0 + ap, oj, ok, false) : zero));
+          }
+        for (int a = 0; a < TX; ++a)
+          for (int bp = -2; bp <= -1; ++bp) {
+            const int oi = oi0 + a;
+            posv.push_back(note(oi < g.Nx ? sp(oi, oj0 + bp, ok, false) : zero));
+          }
+      }
+      if (hi < lo) { lo = 0; hi = -1; }
+      const int64_t len = hi - lo + 1;
+      local_max = std::max(local_max, len);
+      R[1] = (uint32_t)lo;
+      R[2] = (uint32_t)len;
+      // range of the swept cells' indices (the partial moments they write)
+      int64_t clo = INT64_MAX, chi = -1;
+      for (int c = 0; c < np; ++c)
+        for (int b = 0; b < TY; ++b)
+          for (int a = 0; a < TX; ++a)
+            if (swept(o, oi0 + a, oj0 + b, ok0 + c)) {
+              const int64_t cl = phys(o, oi0 + a, oj0 + b, ok0 + c);
+              clo = std::min(clo, cl);
+              chi = std::max(chi, cl);
+            }
+      R[3] = (uint32_t)clo;
+      R[4] = (uint32_t)(chi - clo + 1);
+      local_span = std::max(local_span, chi - clo + 1);
+      auto rel = [&](int64_t p) { return (uint32_t)(p == zero ? len : p - lo); };
+      size_t q = 0;
+      for (int w = 0; w < 2; ++w)
+        for (int s = 0; s < NS; ++s) R[tile_rec_zhalo(NS, w) + s] = rel(posv[q++]);
+      for (int c = 0; c < np; ++c) {
+        uint32_t* P = R + (size_t)(c + 1) * W;
+        const int ok = ok0 + c;
+        uint32_t mask = 0;
+        for (int s = 0; s < NS; ++s) P[tile_rec_slot(s)] = rel(posv[q++]);
+        for (int x = 0; x < 2 * TY; ++x) P[tile_rec_xhalo(NS, x)] = rel(posv[q++]);
+        for (int y = 0; y < 2 * TX; ++y) P[tile_rec_yhalo(NS, TY, y)] = rel(posv[q++]);
+        for (int b = 0; b < TY; ++b)
+          for (int a = 0; a < TX; ++a) {
+            const int s = b * TX + a;
+            const int oi = oi0 + a, oj = oj0 + b;
+            if (!swept(o, oi, oj, ok)) continue;
+            mask |= 1u << s;
+            const int64_t cl = phys(o, oi, oj, ok);
+            const uint32_t cc = g.code[(size_t)cl];
+            const uint32_t flags = ((cc >> (3 * o)) & 7u) | (((cc >> (24 + o)) & 1u) << 3);
+            P[tile_rec_cell(NS, TX, TY, s)] = (uint32_t)cl | flags << 28;
+          }
+        P[0] = mask;
+      }
+    }
+    int64_t cur = gmax.load();
+    while (local_max > cur && !gmax.compare_exchange_weak(cur, local_max)) {
+    }
+    cur = gspan.load();
+    while (local_span > cur && !gspan.compare_exchange_weak(cur, local_span)) {
+    }
+  });
+  g.tl_maxwin = gmax.load();
+  g.tl_maxspan = gspan.load();
+  return true;
+}
+
+void build_wall_positions(Geometry& g, int threads) {
+  const int64_t ns = g.nstore;
   // storage positions the wall kernels read and write
   const size_t nwall = g.wall_cell.size();
   const int K = Geometry::kWallPos;

@@ -107,6 +107,27 @@ struct Geometry {
   // tasks touch (the zero slot aside) lies in [hp_lo, hp_lo + hp_len).
   std::vector<int32_t> hp_lo, hp_len;  // nhp * 8
 
+  // Tiled sweep schedule (build_tile_schedule).  Octant o's oriented grid
+  // (oi, oj, ok in [0, N-1], as above) is cut into tiles of tx * ty * tz
+  // cells; tile (I, J, K) reads upwind values only from itself and from tiles
+  // (I-1, J, K), (I, J-1, K), (I, J, K-1), so the tiles of one diagonal
+  // I + J + K (all octants) are independent and the diagonals run in order.
+  // One wave walks a tile's planes ok = K*tz .. in order, the tx * ty cells of
+  // a plane in raster order (oi fastest).  Each tile has (tz + 1) records of
+  // kTileRecDw dwords: a header (octant, plane count, storage window) with the
+  // two planes below the tile, then one record per plane (see kTileRec* for
+  // the dword positions).  Storage positions are relative to the window's
+  // first index; the zero slot is encoded as the window length.  Only tiles
+  // with at least one swept cell are listed, ordered by diagonal, octant, K,
+  // J, I; tl_off[D] is the first tile of diagonal D.
+  static constexpr int kTileRecDw = 64;
+  int tl_tx = 0, tl_ty = 0, tl_tz = 0, tl_ndiag = 0;
+  int64_t tl_ntile = 0;
+  std::vector<uint32_t> tl_rec;
+  std::vector<int64_t> tl_off;
+  int64_t tl_maxwin = 0;               // largest storage window (cells)
+  int64_t tl_maxspan = 0;              // largest span of a tile's swept cell indices
+
   int64_t cell(int i, int j, int k) const {
     return (int64_t)(i + 1) + (int64_t)(j + 1) * Nxt + (int64_t)(k + 1) * Nxyt;
   }
@@ -118,6 +139,27 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
 
 // Fill the storage order (nstore, pos) and the sweep schedule (hp_task, hp_off).
 void build_sweep_schedule(Geometry& g, int threads);
+
+// Dword positions inside the tiled sweep's records (tile of tx * ty cells per
+// plane, ns = tx * ty slots, slot s = b * tx + a for oriented offsets a, b).
+// Header record: [0] octant | planes << 8, [1] window's first storage index,
+// [2] window length, [3] smallest cell index of a swept cell, [4] the span of
+// the swept cells' indices, [8 + s] plane ok = K*tz - 2, [8 + ns + s] plane
+// K*tz - 1.
+// Plane record: [0] mask of the slots this octant sweeps, [1 + s] the slot's
+// storage position, [1 + ns + 2b + (a' + 2)] x halo a' = -2, -1 of row b,
+// [1 + ns + 2ty + 2a + (b' + 2)] y halo b' = -2, -1 of column a,
+// [1 + 2ns + 2ty + 2tx + s] cell | flags << 28 of a swept slot (flags as in
+// hp_task).
+constexpr int tile_rec_zhalo(int ns, int which) { return 8 + which * ns; }
+constexpr int tile_rec_slot(int s) { return 1 + s; }
+constexpr int tile_rec_xhalo(int ns, int q) { return 1 + ns + q; }
+constexpr int tile_rec_yhalo(int ns, int ty, int q) { return 1 + ns + 2 * ty + q; }
+constexpr int tile_rec_cell(int ns, int tx, int ty, int s) { return 1 + ns + 2 * ty + 2 * tx + s; }
+
+// Fill the tiled sweep schedule (tl_*); needs the storage order.  Returns
+// false when a tile shape does not fit the record (the schedule stays empty).
+bool build_tile_schedule(Geometry& g, int tx, int ty, int tz, int threads);
 
 template <typename F>
 void parallel_for(int n, int threads, F&& f) {

@@ -491,6 +491,209 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) { sweep1_task<T, 
 template <typename T, bool kRecip>
 __global__ __launch_bounds__(64) void k_sweep1_passes(SweepArgs<T> a) { sweep1_task<T, kRecip, false>(a); }
 
+// ------------------------------------------------------------------ tiled sweep
+
+template <typename T>
+struct TileArgs {
+  T* f;                    // right-hand side 0
+  const Mom* mom;
+  double* part;            // [cell][octant][4] partial moments; null = skip
+  const uint32_t* rec;     // this launch's first tile record
+  const double* cx;
+  const double* cy;
+  const double* cz;
+  const double* w;
+  const double* mu;        // per right-hand side
+  double mu0;
+  const ViewArgs<T>* view;
+  double c0, c1, c2, d1;   // 1.5/ds, 2/ds, -0.5/ds, 1/ds
+  int64_t rs_f, rs_mom;    // strides between right-hand sides (elements)
+  int64_t ostride;         // elements of f per octant
+  int32_t nrhs;
+  int32_t ntile;
+  int32_t Nc8;
+  int32_t tz;              // planes per tile (records per tile: tz + 1)
+  uint32_t xcd_per;        // blocks per XCD (grid / 8) when renumbering, else 0
+};
+
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ double lane_f64(uint32_t v, int lane) {
+  const uint64_t lo = lane_u32(v, lane), hi = lane_u32(v, lane + 1);
+  return __builtin_bit_cast(double, lo | hi << 32);
+}
+
+// One tile of one octant per wave: the wave walks the tile's planes in order
+// and, inside a plane, its TX * TY cells in raster order, so every upwind
+// neighbour of a cell is either an earlier cell of the tile (kept in
+// registers: the current plane and the two below it) or a cell of an upwind
+// tile of an earlier launch (loaded with the plane: the x and y halo, and the
+// two planes below the tile).  Cells the octant does not sweep (walls, the
+// inflow layer, ghost planes) keep their value through the sweep and are
+// loaded like the halo; cells without storage read as zero (their position is
+// the window's end, where the buffer returns 0).  The loads of plane c + 1
+// and the record of plane c + 2 are in flight while plane c is computed.
+// Every memory instruction of the plane loop is issued unconditionally (the
+// stores of slots the octant does not sweep go to an offset outside the
+// buffer, which drops them), so that the compiler's count of outstanding
+// memory operations stays exact and a wait for plane c's data never waits for
+// plane c + 1's loads or for earlier stores.
+// Each update is the one of sweep1_task (src/solver.F90:206-214 and
+// analogues), the same operations in the same order, and the partial moments
+// are reduced the same way, so the results are those of the hyperplane sweep
+// bit for bit.  The right-hand side is blockIdx.y.
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(64) void k_tile(TileArgs<T> a) {
+  constexpr int NS = TX * TY;
+  constexpr int NM = (NS + 7) / 8;   // moment vectors: 8 cells (32 B each) per load
+  constexpr int W = Geometry::kTileRecDw;
+  static_assert(tile_rec_cell(NS, TX, TY, NS) <= W && NS <= 32, "tile record");
+  const uint32_t t = xcd_block(a.xcd_per);
+  if (t >= (uint32_t)a.ntile) return;
+  const int lane = threadIdx.x;
+  const int rhs = blockIdx.y;
+  const uint32_t* rec = a.rec + (size_t)t * (uint32_t)(a.tz + 1) * W;
+  const uint32_t hv = rec[lane];
+  const uint32_t h0 = lane_u32(hv, 0);
+  const uint32_t o = h0 & 7u;
+  const int np = (int)(h0 >> 8);
+  const uint32_t lo = lane_u32(hv, 1), len = lane_u32(hv, 2);
+  const uint32_t cell_lo = lane_u32(hv, 3), cell_n = lane_u32(hv, 4);
+  const int Nc8 = a.Nc8;
+  const uint32_t cb = (uint32_t)Nc8 * (uint32_t)sizeof(T);  // bytes per cell
+  const uint32_t oob = len * cb;                            // the zero slot; stores there are dropped
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      a.f + rhs * a.rs_f + (size_t)o * a.ostride + (size_t)lo * (uint32_t)Nc8, 0, (int)oob, kBufferWord3);
+  // partial moments of the tile's cells: [cell - cell_lo][octant][4] doubles
+  const uint32_t poob = cell_n * 256u;
+  const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+      a.part ? a.part + (size_t)rhs * a.rs_mom * 32 + (size_t)cell_lo * 32 : nullptr, 0, a.part ? (int)poob : 0,
+      kBufferWord3);
+  const bool lact = lane < Nc8;
+  const int ll = lact ? lane : 0;
+  const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
+  const uint32_t voff_st = lact ? voff : oob;                 // lanes past the grid store nothing
+  const uint32_t voff_p = (lane & 15) == 0 ? (uint32_t)(lane >> 4) * 8u : poob;
+  const uint32_t* momw = reinterpret_cast<const uint32_t*>(a.mom + rhs * a.rs_mom);
+  const double mu = a.nrhs == 1 ? a.mu0 : a.mu[rhs];
+  const double cxl = a.cx[ll], cyl = a.cy[ll], czl = a.cz[ll], wl = a.w[ll];
+  const double den0 = 0.5 * mu;
+
+  T p2[NS], p1[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    p2[s] = buf_load<T>(rs, voff, lane_u32(hv, tile_rec_zhalo(NS, 0) + s) * cb);
+    p1[s] = buf_load<T>(rs, voff, lane_u32(hv, tile_rec_zhalo(NS, 1) + s) * cb);
+  }
+  // plane c's values (cn, xn, yn, mn) are issued one plane ahead
+  T cn[NS], xn[2 * TY], yn[2 * TX];
+  uint32_t mn[NM];
+  auto issue = [&](uint32_t r) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) cn[s] = buf_load<T>(rs, voff, lane_u32(r, tile_rec_slot(s)) * cb);
+#pragma unroll
+    for (int q = 0; q < 2 * TY; ++q) xn[q] = buf_load<T>(rs, voff, lane_u32(r, tile_rec_xhalo(NS, q)) * cb);
+#pragma unroll
+    for (int q = 0; q < 2 * TX; ++q) yn[q] = buf_load<T>(rs, voff, lane_u32(r, tile_rec_yhalo(NS, TY, q)) * cb);
+    // lane q of moment vector v holds dword q % 8 of the moments of slot
+    // 8v + q / 8 (slots the octant does not sweep carry cell 0: a harmless read)
+#pragma unroll
+    for (int v = 0; v < NM; ++v) {
+      const int slot = 8 * v + (lane >> 3);
+      const uint32_t cw = (uint32_t)__shfl((int)r, tile_rec_cell(NS, TX, TY, slot < NS ? slot : 0), kWave);
+      mn[v] = momw[(size_t)(cw & 0x0fffffffu) * 8 + (lane & 7)];
+    }
+  };
+  // records of the planes past the last one are read too (zeros, or the last
+  // plane's record again): the loads they feed are harmless and keep the
+  // loop's memory instructions unconditional
+  const int last = a.tz;
+  uint32_t rcur = rec[W + lane];
+  issue(rcur);
+  uint32_t rnext = rec[(size_t)(np > 1 ? 2 : 1) * W + lane];
+  for (int c = 0; c < np; ++c) {
+    T cc[NS], xh[2 * TY], yh[2 * TX];
+    uint32_t mm[NM];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) cc[s] = cn[s];
+#pragma unroll
+    for (int q = 0; q < 2 * TY; ++q) xh[q] = xn[q];
+#pragma unroll
+    for (int q = 0; q < 2 * TX; ++q) yh[q] = yn[q];
+#pragma unroll
+    for (int v = 0; v < NM; ++v) mm[v] = mn[v];
+    const uint32_t r = rcur;
+    issue(rnext);
+    rcur = rnext;
+    rnext = rec[(size_t)(c + 3 < last ? c + 3 : last) * W + lane];
+    const uint32_t mask = lane_u32(r, 0);
+#pragma unroll
+    for (int b = 0; b < TY; ++b)
+#pragma unroll
+      for (int x = 0; x < TX; ++x) {
+        const int s = b * TX + x;
+        const bool sw = (mask >> s) & 1u;
+        const uint32_t cw = lane_u32(r, tile_rec_cell(NS, TX, TY, s));
+        const uint32_t cell = cw & 0x0fffffffu, flg = cw >> 28;
+        double zz = 0.0;
+        if (sw) {
+          double nb[6];
+          nb[0] = x >= 1 ? (double)cc[s - 1] : (double)xh[2 * b + 1];
+          nb[1] = x >= 2 ? (double)cc[s - 2] : (double)xh[2 * b + x];
+          nb[2] = b >= 1 ? (double)cc[s - TX] : (double)yh[2 * x + 1];
+          nb[3] = b >= 2 ? (double)cc[s - 2 * TX] : (double)yh[2 * x + b];
+          nb[4] = (double)p1[s];
+          nb[5] = (double)p2[s];
+          const double fp = (double)cc[s];
+          const uint32_t mv = mm[s >> 3];
+          const int ml = (s & 7) * 8;
+          const double mr = lane_f64(mv, ml), mux = lane_f64(mv, ml + 2), muy = lane_f64(mv, ml + 4),
+                       muz = lane_f64(mv, ml + 6);
+          if (flg & 8u) {  // reads a wall slot the reference switched
+            const ViewFix<T> vf = view_fix(a.view, cell, (int)o, ll, rhs);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (vf.mask & (1u << q)) nb[2 + q] = (double)vf.v[q];
+          }
+          const bool fx = flg & 1u, fy = flg & 2u, fz = flg & 4u;
+          const double mx = osx((int)o) > 0 ? mux : -mux;
+          const double my = osy((int)o) > 0 ? muy : -muy;
+          const double mz = osz((int)o) > 0 ? muz : -muz;
+          double tt = cxl * mx;
+          tt = tt + cyl * my;
+          tt = tt + czl * mz;
+          const double feq = wl * (mr + 1.0 * tt);
+          double num = mu * (feq - 0.5 * fp);
+          num = num + cxl * (fx ? a.d1 : a.c1) * nb[0];
+          num = num + cxl * (fx ? 0.0 : a.c2) * nb[1];
+          num = num + cyl * (fy ? a.d1 : a.c1) * nb[2];
+          num = num + cyl * (fy ? 0.0 : a.c2) * nb[3];
+          num = num + czl * (fz ? a.d1 : a.c1) * nb[4];
+          num = num + czl * (fz ? 0.0 : a.c2) * nb[5];
+          double den = den0 + cxl * (fx ? a.d1 : a.c0);
+          den = den + cyl * (fy ? a.d1 : a.c0);
+          den = den + czl * (fz ? a.d1 : a.c0);
+          const T fnew = (T)(num / den);
+          cc[s] = fnew;
+          if (a.part) {
+            const double fv = lact ? (double)fnew : 0.0;
+            const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv);
+            const uint32_t q = (uint32_t)lane >> 4;
+            zz = ((kOctNeg >> (4 * o + q)) & 1u) ? -z : z;
+          }
+        }
+        buf_store(cc[s], rs, voff_st, sw ? lane_u32(r, tile_rec_slot(s)) * cb : oob);
+        buf_store(zz, rp, voff_p, sw ? ((cell - cell_lo) * 8u + o) * 32u : poob);
+      }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      p2[s] = p1[s];
+      p1[s] = cc[s];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ switch commit
 
 // Final state of the switched wall slots after a sweep: the z switch if it
@@ -1134,6 +1337,16 @@ SweepKernel<T> sweep_kernel(int block, bool recip, int rb) {
   }
 }
 
+// The tiled sweep kernel for a tile cross-section (null: not compiled).
+template <typename T>
+void (*tile_kernel(int tx, int ty))(TileArgs<T>) {
+  if (tx == 4 && ty == 4) return k_tile<T, 4, 4>;
+  if (tx == 4 && ty == 2) return k_tile<T, 4, 2>;
+  if (tx == 8 && ty == 2) return k_tile<T, 8, 2>;
+  if (tx == 2 && ty == 2) return k_tile<T, 2, 2>;
+  return nullptr;
+}
+
 // Enqueue nsteps iterations on the context's stream.  With `timed`, four
 // events per step (at most kMaxTimedSteps) bracket the sweep, boundary and
 // moment phases; the caller synchronises and reads them.
@@ -1165,10 +1378,42 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
     HIP_TRY(hipEventCreate(&e));
     c.ev.push_back(e);
   }
+  TileArgs<T> ta{};
+  void (*ktile)(TileArgs<T>) = nullptr;
+  if (c.sweep_tiles) {
+    ta.f = st.f;
+    ta.mom = c.mom;
+    ta.part = c.exact ? nullptr : c.part;
+    ta.cx = c.dcx; ta.cy = c.dcy; ta.cz = c.dcz; ta.w = c.dw;
+    ta.mu = c.d_mu;
+    ta.mu0 = c.mu_r[0];
+    ta.view = sa.view;
+    ta.c0 = sa.c0; ta.c1 = sa.c1; ta.c2 = sa.c2; ta.d1 = sa.d1;
+    ta.rs_f = d.rs_f;
+    ta.rs_mom = d.Ntotal;
+    ta.ostride = d.ostride;
+    ta.nrhs = c.nrhs;
+    ta.Nc8 = d.Nc8;
+    ta.tz = g.tl_tz;
+    ktile = tile_kernel<T>(g.tl_tx, g.tl_ty);
+    if (!ktile) return fail(PIKS_E_ARG, "no tiled sweep kernel for this tile shape");
+  }
   for (int step = 0; step < nsteps; ++step) {
     const bool tm = step < timed;
     if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 0], c.stream));
-    for (int h = 0; h < g.nhp; ++h) {
+    if (c.sweep_tiles) {
+      const size_t per_tile = (size_t)(g.tl_tz + 1) * Geometry::kTileRecDw;
+      for (int dg = 0; dg < g.tl_ndiag; ++dg) {
+        const int64_t n = g.tl_off[(size_t)dg + 1] - g.tl_off[(size_t)dg];
+        if (n == 0) continue;
+        ta.ntile = (int32_t)n;
+        ta.rec = c.tl_rec + (size_t)g.tl_off[(size_t)dg] * per_tile;
+        const int64_t blocks = (n + 7) / 8 * 8;
+        ta.xcd_per = c.xcd_remap ? (uint32_t)(blocks / 8) : 0u;
+        hipLaunchKernelGGL(ktile, dim3((unsigned)blocks, R), dim3(64), 0, c.stream, ta);
+      }
+    }
+    for (int h = 0; h < (c.sweep_tiles ? 0 : g.nhp); ++h) {
       sa.ntask = (int32_t)(g.hp_off[h + 1] - g.hp_off[h]);
       if (sa.ntask == 0) continue;
       sa.tasks = reinterpret_cast<const TaskRec*>(c.hp_task) + g.hp_off[h];
@@ -1301,6 +1546,13 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
     c->sweep_block = v;
   }
 
+  if (const char* b = std::getenv("PIKS_SWEEP_TILES")) c->sweep_tiles = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_TILE")) {
+    int tx = 0, ty = 0, tz = 0;
+    if (std::sscanf(b, "%dx%dx%d", &tx, &ty, &tz) != 3 || !tile_kernel<double>(tx, ty) || tz < 2 || tz > 256)
+      return fail(PIKS_E_ARG, "PIKS_TILE must be TXxTYxTZ with TXxTY one of 4x4, 4x2, 8x2, 2x2 and TZ in 2..256");
+    c->tile_tx = tx; c->tile_ty = ty; c->tile_tz = tz;
+  }
   if (const char* b = std::getenv("PIKS_GRAPH")) c->use_graph = std::atoi(b) != 0;
   if (const char* b = std::getenv("PIKS_SWEEP_RECIP")) c->sweep_recip = std::atoi(b) != 0;
   if (const char* b = std::getenv("PIKS_SWEEP_GENERIC")) c->sweep_generic = std::atoi(b) != 0;
@@ -1312,6 +1564,22 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   c->d.rs_fw = (int64_t)c->n_fw;
   c->d.rs_buf = (int64_t)c->n_buf;
   c->d.rs_snap = (int64_t)c->n_snap;
+  // the tiled sweep serves grids of at most 64 velocities per octant and the
+  // generic kernel is a forced alternative
+  if (c->d.Nc8 > kWave || c->sweep_generic) c->sweep_tiles = false;
+  if (c->sweep_tiles) {
+    Geometry& gt = c->g;
+    if (!build_tile_schedule(gt, c->tile_tx, c->tile_ty, c->tile_tz, host_threads()))
+      return fail(PIKS_E_ARG, "tile shape does not fit the tile record");
+    // every buffer offset of a tile (bytes, the zero slot included, and the
+    // partial moments of its cells) is 32-bit
+    if ((gt.tl_maxwin + 1) * c->d.Nc8 * (int64_t)sizeof(double) >= ((int64_t)1 << 31) ||
+        (gt.tl_maxspan + 1) * 256 >= ((int64_t)1 << 31)) {
+      c->sweep_tiles = false;
+      gt.tl_rec.clear();
+      gt.tl_rec.shrink_to_fit();
+    }
+  }
   // the sweep addresses one hyperplane's storage window with 32-bit offsets
   int64_t maxwin = 0;
   for (int32_t v : g.hp_len) maxwin = std::max<int64_t>(maxwin, v);
@@ -1346,7 +1614,8 @@ size_t ctx_device_bytes(const Ctx& c) {
                        g.uncoupled.size() * 4 +
                        g.coupled.size() * 4 * 7 + g.snap_cell.size() * 4 + g.commit_walls.size() * 4;
   const size_t fluid = (size_t)(g.fluid_interior.size() + g.fluid_boundary.size()) * 8;
-  return state64 + state32 + per_cell + walls + fluid + g.hp_task.size() * 4 + g.hp_lo.size() * 8 +
+  const size_t sched = c.sweep_tiles ? g.tl_rec.size() * 4 : g.hp_task.size() * 4 + g.hp_lo.size() * 16;
+  return state64 + state32 + per_cell + walls + fluid + sched +
          R * ((size_t)g.Ny * g.Nz * 8 + 8) + 4 * c.cx.size() * 8 + 64 * 256;
 }
 
@@ -1423,15 +1692,19 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
   std::vector<int32_t> all(g.fluid_interior);
   all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());
   if ((rc = upload(c, &c.fluid_all, all))) return rc;
-  if ((rc = upload(c, &c.hp_task, g.hp_task))) return rc;
-  std::vector<SweepWin> win(g.hp_lo.size());
-  for (size_t q = 0; q < win.size(); ++q) {
-    const int o = (int)(q % 8);
-    win[q].off = (int64_t)o * c.d.ostride + (int64_t)g.hp_lo[q] * c.d.Nc8;
-    win[q].len = (uint32_t)((int64_t)g.hp_len[q] * c.d.Nc8);
-    win[q].pad = 0;
+  if (c.sweep_tiles) {
+    if ((rc = upload(c, &c.tl_rec, g.tl_rec))) return rc;
+  } else {
+    if ((rc = upload(c, &c.hp_task, g.hp_task))) return rc;
+    std::vector<SweepWin> win(g.hp_lo.size());
+    for (size_t q = 0; q < win.size(); ++q) {
+      const int o = (int)(q % 8);
+      win[q].off = (int64_t)o * c.d.ostride + (int64_t)g.hp_lo[q] * c.d.Nc8;
+      win[q].len = (uint32_t)((int64_t)g.hp_len[q] * c.d.Nc8);
+      win[q].pad = 0;
+    }
+    if ((rc = upload(c, &c.hp_win, win))) return rc;
   }
-  if ((rc = upload(c, &c.hp_win, win))) return rc;
   for (int F = 0; F < 4; ++F) {
     if ((rc = upload(c, &c.pos[F], g.pos[F]))) return rc;
     c.d.pos[F] = c.pos[F];
@@ -1448,7 +1721,10 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
       c.view_args[1] = p32;
     }
   }
-  if (c.own_stream) HIP_TRY(hipStreamSynchronize(c.stream));
+  // the uploads read host vectors, some local to this function: wait for
+  // them here so that no copy outlives its source, whatever HIP does with
+  // pageable memory (on a caller's stream this waits for that stream only)
+  HIP_TRY(hipStreamSynchronize(c.stream));
   return 0;
 }
 
@@ -1682,9 +1958,10 @@ int piks_get_macro_rhs(void* ctx, int32_t rhs, double* rho, double* ux, double* 
 int piks_chk_converge(void* ctx, double out[3]) {
   if (!ctx || !out) return fail(PIKS_E_ARG, "null argument");
   Ctx& c = *static_cast<Ctx*>(ctx);
-  double m = 0.0;
-  if (int rc = mass_flow_sync(c, &m)) return rc;
-  ctx_check(c, 0, m, out);  // src/output.F90:151-155
+  // one mass flow per right-hand side (a multi context reports the first)
+  std::vector<double> m((size_t)c.nrhs);
+  if (int rc = mass_flow_sync(c, m.data())) return rc;
+  ctx_check(c, 0, m[0], out);  // src/output.F90:151-155
   return 0;
 }
 

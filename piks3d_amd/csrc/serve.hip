@@ -83,6 +83,7 @@ struct Server {
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
   int64_t next_ticket = 1;
   int pending = 0;  // submitted, not finished
+  int waiters = 0;  // threads inside piks_serve_wait / piks_serve_drain
   bool stop = false;
   std::vector<Slot> slots;
   std::vector<std::thread> workers;
@@ -90,7 +91,10 @@ struct Server {
   std::string sched_error;
 };
 
+// Publish a request's end (under the lock).  A request ends once: the server's
+// teardown may fail a request whose last chunk the scheduler is retiring.
 void finish(Server& s, Request& r, int status) {
+  if (r.status == PIKS_REQ_DONE || r.status == PIKS_REQ_FAILED) return;
   r.status = status;
   r.t_done = Clock::now();
   --s.pending;
@@ -147,61 +151,103 @@ int enqueue_chunk(Slot& sl, Request& r) {
   return 0;
 }
 
+// The scheduler thread alone touches the slots, the contexts of running
+// requests and their solve state; s.mu guards the queues, the published
+// result fields, the memory accounting and the request status.  Device work
+// (allocation, upload, enqueue, teardown) runs without the lock, so that
+// submit, poll, wait and the preparation workers never wait behind it.
+
+// Release a failed running request's slot and memory (called without the lock).
 void fail_request(Server& s, Slot& sl, Request& r, int rc) {
-  r.error_code = rc;
-  r.error_msg = piks_last_error();
+  const std::string msg = piks_last_error();
   ctx_destroy(r.ctx);
+  std::lock_guard<std::mutex> lk(s.mu);
   r.ctx = nullptr;
+  r.error_code = rc;
+  r.error_msg = msg;
   s.in_use -= r.bytes;
   sl.req = nullptr;
   finish(s, r, PIKS_REQ_FAILED);
 }
 
-// A chunk finished: evaluate the check like the reference's loop.
+// A chunk finished: evaluate the check like the reference's loop (called
+// without the lock).
 void on_chunk_done(Server& s, Slot& sl, Request& r) {
   Ctx& c = *r.ctx;
-  r.step += r.chunk;
-  if (r.refined) r.steps32 += r.chunk;
+  const int step = r.step + r.chunk;
+  const int steps32 = r.steps32 + (r.refined ? r.chunk : 0);
   const int chk = c.prm.chk_converge_step;
-  bool done = false;
-  if (r.step % chk == 0) {
+  bool done = false, converged = false, to_fp64 = false;
+  double mass = r.mass, perm = r.perm, error = r.error;
+  if (step % chk == 0) {
     double out[3];
     ctx_check(c, 0, ctx_mass(c, sl.plane), out);
-    r.mass = out[0];
-    r.perm = out[1];
-    r.error = out[2];
+    mass = out[0];
+    perm = out[1];
+    error = out[2];
     if (r.refined) {
       // FP32 has gone as far as it can: below the switch level, or no
       // improvement for three checks; finish in FP64
-      if (r.error < r.best) {
-        r.best = r.error;
+      if (error < r.best) {
+        r.best = error;
         r.stalled = 0;
       } else {
         ++r.stalled;
       }
       // the last max(3 checks, max_step / 5) iterations always run in FP64
       const int tail = std::max(3 * chk, r.max_step / 5);
-      if (r.error <= std::max(r.refine_switch, r.eps) || r.stalled >= 3 || r.step >= r.max_step - tail) {
-        if (int rc = ctx_set_phase(c, PIKS_PREC_FP64)) return fail_request(s, sl, r, rc);
-        r.refined = false;
-        r.error = 1.0;
-      }
-    } else if (r.error <= r.eps) {
-      r.converged = true;
+      if (error <= std::max(r.refine_switch, r.eps) || r.stalled >= 3 || step >= r.max_step - tail) to_fp64 = true;
+    } else if (error <= r.eps) {
+      converged = true;
       done = true;
     }
   }
-  if (r.step >= r.max_step) done = true;
+  if (step >= r.max_step) done = true;
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    r.step = step;
+    r.steps32 = steps32;
+    r.mass = mass;
+    r.perm = perm;
+    r.error = error;
+    r.converged = converged;
+  }
+  if (to_fp64) {
+    if (int rc = ctx_set_phase(c, PIKS_PREC_FP64)) return fail_request(s, sl, r, rc);
+    r.refined = false;
+    std::lock_guard<std::mutex> lk(s.mu);
+    r.error = 1.0;
+  }
   if (!done) {
     if (int rc = enqueue_chunk(sl, r)) fail_request(s, sl, r, rc);
     return;
   }
-  r.mass = c.mass;
+  const double final_mass = c.mass;
   ctx_destroy(r.ctx);  // stream-ordered frees: no device-wide wait
+  std::lock_guard<std::mutex> lk(s.mu);
   r.ctx = nullptr;
+  r.mass = final_mass;
   s.in_use -= r.bytes;
   sl.req = nullptr;
   finish(s, r, PIKS_REQ_DONE);
+}
+
+// Device set-up of an admitted request on its slot (called without the lock).
+void start_request(Server& s, Slot& sl, Request& r) {
+  const size_t need = (size_t)r.ctx->g.Ny * r.ctx->g.Nz;
+  if (need > sl.plane_cap) {
+    if (sl.plane) (void)hipHostFree(sl.plane);
+    sl.plane = nullptr;
+    sl.plane_cap = 0;
+    if (hipHostMalloc((void**)&sl.plane, need * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      set_error(PIKS_E_MEM, "pinned host allocation failed");
+      return fail_request(s, sl, r, PIKS_E_MEM);
+    }
+    sl.plane_cap = need;
+  }
+  int rc = ctx_create_device(*r.ctx, sl.stream);
+  if (rc == 0) rc = enqueue_chunk(sl, r);
+  if (rc) fail_request(s, sl, r, rc);
 }
 
 void scheduler(Server* s) {
@@ -210,64 +256,51 @@ void scheduler(Server* s) {
     s->sched_error = "hipSetDevice failed";
     return;
   }
+  std::vector<std::pair<Slot*, Request*>> admit;
   for (;;) {
-    bool progressed = false;
-    std::unique_lock<std::mutex> lk(s->mu);
-    if (s->stop) return;
-    // admit in arrival order while a slot and the memory are free
-    while (!s->ready_q.empty()) {
-      Request* r = s->ready_q.front();
-      Slot* free_slot = nullptr;
-      for (auto& sl : s->slots)
-        if (!sl.req) {
-          free_slot = &sl;
-          break;
-        }
-      if (!free_slot || s->in_use + r->bytes > s->budget) break;
-      s->ready_q.pop_front();
-      Slot& sl = *free_slot;
-      const size_t need = (size_t)r->ctx->g.Ny * r->ctx->g.Nz;
-      if (need > sl.plane_cap) {
-        if (sl.plane) (void)hipHostFree(sl.plane);
-        sl.plane = nullptr;
-        if (hipHostMalloc((void**)&sl.plane, need * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-          sl.plane_cap = 0;
-          r->error_code = PIKS_E_MEM;
-          r->error_msg = "pinned host allocation failed";
-          ctx_destroy(r->ctx);
-          r->ctx = nullptr;
-          finish(*s, *r, PIKS_REQ_FAILED);
-          continue;
-        }
-        sl.plane_cap = need;
+    admit.clear();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->stop) return;
+      // admit in arrival order while a slot and the memory are free
+      while (!s->ready_q.empty()) {
+        Request* r = s->ready_q.front();
+        Slot* free_slot = nullptr;
+        for (auto& sl : s->slots)
+          if (!sl.req) {
+            free_slot = &sl;
+            break;
+          }
+        if (!free_slot || s->in_use + r->bytes > s->budget) break;
+        s->ready_q.pop_front();
+        r->slot = (int)(free_slot - s->slots.data());
+        r->t_admit = Clock::now();
+        r->status = PIKS_REQ_RUNNING;
+        r->refined = r->prm.precision == PIKS_PREC_FP32_REFINED;
+        s->in_use += r->bytes;
+        free_slot->req = r;
+        admit.emplace_back(free_slot, r);
       }
-      r->slot = (int)(&sl - s->slots.data());
-      r->t_admit = Clock::now();
-      r->status = PIKS_REQ_RUNNING;
-      r->refined = r->prm.precision == PIKS_PREC_FP32_REFINED;
-      s->in_use += r->bytes;
-      sl.req = r;
-      int rc = ctx_create_device(*r->ctx, sl.stream);
-      if (rc == 0) rc = enqueue_chunk(sl, *r);
-      if (rc) fail_request(*s, sl, *r, rc);
-      progressed = true;
     }
+    bool progressed = !admit.empty();
+    for (auto& a : admit) start_request(*s, *a.first, *a.second);
     // retire or continue the running requests whose chunk has finished
     for (auto& sl : s->slots) {
       if (!sl.req) continue;
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
+      progressed = true;
       if (q != hipSuccess) {
         set_error(PIKS_E_HIP, std::string("solve failed on the device: ") + hipGetErrorString(q));
         fail_request(*s, sl, *sl.req, PIKS_E_HIP);
         continue;
       }
       on_chunk_done(*s, sl, *sl.req);
-      progressed = true;
     }
     if (!progressed) {
       bool running = false;
       for (auto& sl : s->slots) running |= sl.req != nullptr;
+      std::unique_lock<std::mutex> lk(s->mu);
       if (running)
         s->cv_sched.wait_for(lk, std::chrono::microseconds(50));
       else
@@ -370,8 +403,11 @@ int piks_serve_wait(void* server, int64_t ticket, piks_serve_result* out) {
   auto it = s.reqs.find(ticket);
   if (it == s.reqs.end()) return set_error(PIKS_E_ARG, "unknown ticket " + std::to_string(ticket));
   Request& r = *it->second;
+  ++s.waiters;
   s.cv_done.wait(lk, [&] { return r.status == PIKS_REQ_DONE || r.status == PIKS_REQ_FAILED; });
   fill_result(r, out);
+  --s.waiters;
+  s.cv_done.notify_all();
   return 0;
 }
 
@@ -379,7 +415,10 @@ int piks_serve_drain(void* server) {
   if (!server) return set_error(PIKS_E_ARG, "null argument");
   Server& s = *static_cast<Server*>(server);
   std::unique_lock<std::mutex> lk(s.mu);
+  ++s.waiters;
   s.cv_done.wait(lk, [&] { return s.pending == 0; });
+  --s.waiters;
+  s.cv_done.notify_all();
   return 0;
 }
 
@@ -387,8 +426,20 @@ int piks_serve_destroy(void* server) {
   if (!server) return 0;
   Server* s = static_cast<Server*>(server);
   {
-    std::lock_guard<std::mutex> lk(s->mu);
+    // every request that has not finished fails with PIKS_E_STATE, which
+    // releases the threads blocked in wait and drain; they leave before the
+    // server is freed
+    std::unique_lock<std::mutex> lk(s->mu);
     s->stop = true;
+    for (auto& kv : s->reqs) {
+      Request& r = *kv.second;
+      if (r.status == PIKS_REQ_DONE || r.status == PIKS_REQ_FAILED) continue;
+      r.error_code = PIKS_E_STATE;
+      r.error_msg = "the server was destroyed";
+      finish(*s, r, PIKS_REQ_FAILED);
+    }
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
   }
   s->cv_prep.notify_all();
   s->cv_sched.notify_all();

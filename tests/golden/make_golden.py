@@ -99,6 +99,15 @@ LONG_CASES = {
                     dict(fluid_layer=4, nc_fundamental=4, kn=0.1, press_drop=1.0, accom=1.0,
                          wall_ext_order=2, chk_converge_step=10, max_step=100000, eps=1e-6),
                     ["mfinal"], "history"),
+    # the 200^3 sphere pack of the headline throughput workload (seed 2024),
+    # 8^3 velocities, Kn 1, wallExtOrder 2: 30 iterations, checked every 10.
+    # The image is regenerated from its seed (its SHA-256 is in the fixture)
+    # and only the check lines, the final Ux plane x = 2 and the moment sums
+    # are kept.
+    "c2_spheres200": (lambda: images.sphere_pack(200, 0.40, 4.0, 10.0, seed=2024),
+                      dict(fluid_layer=4, nc_fundamental=4, kn=1.0, press_drop=1.0, accom=1.0,
+                           wall_ext_order=2, chk_converge_step=10, max_step=30, eps=0.0),
+                      ["mfinal"], "history_compact"),
     # the reference's own tutorial (conf/createSphere.F90 image), 100 steps
     "tutorial100": (lambda: images.quarter_sphere(100, 100, 100, 38.0),
                     dict(fluid_layer=0, nc_fundamental=4, kn=0.1, press_drop=0.1, accom=1.0,
@@ -138,8 +147,11 @@ def run_case(name, factory, over, dump_args, subset, threads=1):
     Ntotal = int(sc["grid"][6])
     Nc8 = int(sc["nc"][1])
     ncorner = int(sc["counts"][2])
+    compact = subset == "history_compact"
+    if compact:
+        subset = "history"
     fx = dict(
-        image=img, nx=nx, ny=ny, nz=nz,
+        nx=nx, ny=ny, nz=nz,
         params=np.array([p.wall_ext_order, p.fluid_layer, p.nc_fundamental, int(p.half_range),
                          p.chk_converge_step, p.max_step], dtype=np.int64),
         fparams=np.array([p.ref_l, p.kn, p.press_drop, p.accom, p.eps]),
@@ -150,8 +162,15 @@ def run_case(name, factory, over, dump_args, subset, threads=1):
                         dtype=np.int64),
         stdout=r.stdout,
     )
-    if subset != "velocity":
-        fx.update(label=rd("setup_image.bin", np.int32).astype(np.int8),
+    if compact:
+        fx["image_sha256"] = hashlib.sha256(np.ascontiguousarray(img).tobytes()).hexdigest()
+    else:
+        fx["image"] = img
+    label = rd("setup_image.bin", np.int32).astype(np.int8) if subset != "velocity" else None
+    if compact:
+        fx["label_sha256"] = hashlib.sha256(label.tobytes()).hexdigest()
+    elif subset != "velocity":
+        fx.update(label=label,
                   which_corner=rd("setup_which_corner.bin", np.int32),
                   vecwall=rd("setup_vecWall.bin", np.int32),
                   extcoef=rd("setup_extCoef.bin"),
@@ -166,7 +185,7 @@ def run_case(name, factory, over, dump_args, subset, threads=1):
             mac = np.stack([rd("%s_%s.bin" % (q, tag)) for q in ("rho", "ux", "uy", "uz")])
             if subset == "history":
                 # interior fluid cells only, as float64
-                fluid = fx["label"] == 0
+                fluid = label == 0
                 fx["macro_%s_fluidsum" % tag] = mac[:, fluid].sum(axis=1)
                 fx["macro_%s_fluidsq" % tag] = (mac[:, fluid] ** 2).sum(axis=1)
                 Nxt = nx + 2 * fl + 4
@@ -201,8 +220,10 @@ def main(argv):
             run_case(name, factory, dict(over), list(dump_args), subset)
         elif name in LONG_CASES:
             factory, over, dump_args, subset = LONG_CASES[name]
-            run_case(name, factory, dict(over), list(dump_args), subset,
-                     threads=int(os.environ.get("GOLDEN_THREADS", "8")))
+            # the sphere packs have walls that read other walls: one thread
+            # gives the reference's serial wall order
+            threads = 1 if name == "c2_spheres200" else int(os.environ.get("GOLDEN_THREADS", "8"))
+            run_case(name, factory, dict(over), list(dump_args), subset, threads=threads)
         else:
             raise SystemExit("unknown case " + name)
         print("wrote", name, flush=True)

@@ -48,3 +48,18 @@ def test_refined_multi_matches_fp64_multi():
     assert 0 < r32["steps32"] < r32["steps"] == r64["steps"] == 500
     rel = np.abs(np.asarray(r32["permeability"]) - r64["permeability"]) / np.abs(r64["permeability"])
     assert np.all(rel <= 1e-8), rel
+
+
+def test_single_rhs_calls_on_a_multi_context():
+    # piks_chk_converge and piks_mass_flow on a context with several
+    # right-hand sides report the first one (and must not write past it)
+    z = load_golden("boxes_n2_a05_e1")
+    p = params_from_golden(z)
+    with MultiSolver(p, z["image"], KNS) as ms:
+        ms.iterate(3)
+        m0 = ms.mass_flow()
+        m, k, e = ms.chk_converge()
+        m_all, k_all, e_all = ms.chk_converge_all()
+    assert m0 == m == m_all[0]
+    assert k == k_all[0]
+    assert e_all[0] == 0.0                    # the stored mass was updated by chk_converge
