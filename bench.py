@@ -83,6 +83,78 @@ def cpu_baseline(p, img, nfluid, nc, threads, iters=2):
                       "(flang -O2 -fopenmp, 1 MPI rank), loop wall time %.3g s" % (iters, wall)}
 
 
+def physical_cores():
+    """Physical cores this process may run on: distinct (package, core) pairs
+    of /proc/cpuinfo among the CPUs of the affinity mask (the logical count
+    when the file gives no topology)."""
+    try:
+        allowed = os.sched_getaffinity(0)
+    except AttributeError:
+        allowed = set(range(os.cpu_count() or 1))
+    cores, cpu, pkg = set(), None, None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    cpu, pkg = int(v), None
+                elif k == "physical id":
+                    pkg = v
+                elif k == "core id" and cpu in allowed:
+                    cores.add((pkg, v))
+    except OSError:
+        pass
+    return len(cores) if cores else len(allowed)
+
+
+def refined_measurement(p, img, steps, warmup, block, k64, fp64_ms_per_step):
+    """The FP64 defect correction on the same workload: `warmup` plain FP64
+    iterations, then `steps` iterations in blocks of `block` (one FP64
+    residual iteration, block - 1 FP32 correction iterations).  Returns the
+    whole-iteration rate of the timed steps, the correction sweep's roofline
+    at its algorithmic 12 B per update (read the correction and the residual,
+    write the correction) and K against the plain FP64 run at equal
+    iteration counts."""
+    from piks3d_amd import Solver
+    os.environ["PIKS_DC_BLOCK"] = "1"
+    with Solver(p, img, precision="fp32") as s:
+        nf, nc = s.info.nfluid, s.info.nc8 * 8
+        s.iterate(warmup)
+        s.chk_converge()
+        os.environ.pop("PIKS_DC_BLOCK")
+        # the block length from here on (the library's policy would pick it
+        # from the check; fixed here so that the measurement is repeatable)
+        s.set_correction_block(block)
+        t0 = time.perf_counter()
+        s.iterate(steps)
+        t1 = time.perf_counter()
+        ms = s.timing()
+        m, k, err = s.chk_converge()
+        # correction iterations alone: the block's iterations 2..block
+        corr = []
+        for q in range(block):
+            s.iterate(1)
+            t = s.timing()
+            corr.append(t)
+        n32 = s.get_info().steps_fp32
+    corr = sorted(corr, key=lambda t: t[3])[:-1]  # drop the block's FP64 iteration (the slowest)
+    sweep_ms = sum(t[0] for t in corr) / len(corr)
+    it_ms = sum(t[3] for t in corr) / len(corr)
+    upd = nf * nc
+    ach = upd * 12 / (sweep_ms / 1e3) / 1e9
+    dt = t1 - t0
+    return {
+        "block": block, "value": upd * steps / dt, "unit": "updates/s", "ms_per_step": dt / steps * 1e3,
+        "speedup_vs_fp64": fp64_ms_per_step / (dt / steps * 1e3),
+        "correction_iteration_ms": round(it_ms, 4), "correction_sweep_ms": round(sweep_ms, 4),
+        "correction_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_update": 12,
+                                "kernel": "k_sweep1<float, false, 2>"},
+        "k_rel_diff_vs_fp64": abs(k - k64) / abs(k64), "steps_compared": warmup + steps,
+        "fp32_iterations": int(n32)}
+
+
 def traffic_from_profiles(workload_name):
     path = os.path.join(HERE, "profiles", "traffic_%s.json" % workload_name)
     if os.path.exists(path):
@@ -364,7 +436,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: every physical core of the affinity mask)")
+    ap.add_argument("--dc-block", type=int, default=20,
+                    help="block length of the defect-correction measurement in the bench line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -390,6 +465,8 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
+    if args.cpu_threads <= 0:
+        args.cpu_threads = physical_cores()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -456,6 +533,15 @@ def main():
             args.steps, elapsed, value, ms[0] / args.steps, ms[1] / args.steps, ms[2] / args.steps,
             args.warmup + args.steps, k))
     s.close()
+    refined = None
+    if args.precision == "fp64" and args.dc_block > 1:
+        refined = refined_measurement(p, img, args.steps, args.warmup, args.dc_block, k,
+                                      elapsed / args.steps * 1e3)
+        log("defect correction, blocks of %d: %.4g updates/s (%.2fx FP64), correction iteration %.3f ms "
+            "(sweep %.3f ms, %.1f %% of HBM at 12 B/update), K rel diff %.2e" % (
+                args.dc_block, refined["value"], refined["speedup_vs_fp64"], refined["correction_iteration_ms"],
+                refined["correction_sweep_ms"], 100 * refined["correction_roofline"]["frac"],
+                refined["k_rel_diff_vs_fp64"]))
 
     wl = "spheres200_phi040_kn1_v8"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -478,6 +564,9 @@ def main():
                    "domain": [info.nx, info.ny, info.nz], "fluid_cells": info.nfluid,
                    "velocities": nc, "kn": p.kn, "precision": args.precision},
         "roofline": roof,
+        "whole_iteration_roofline": {"achieved": round(updates * bytes_per_update / elapsed / 1e9, 1),
+                                     "frac": round(updates * bytes_per_update / elapsed / 1e9 / HBM_PEAK_GBS, 4)},
+        "refined": refined,
         "cpu_baseline": cpu,
         "phase_ms_per_step": [round(x / args.steps, 4) for x in ms[:3]],
     }

@@ -40,7 +40,8 @@ extern "C" {
 #define PIKS_E_STATE (-7)    /* call not valid in this state */
 
 #define PIKS_PREC_FP64 0       /* f stored and swept in double */
-#define PIKS_PREC_FP32_REFINED 1 /* f stored in FP32 for the bulk of the iterations, FP64 to finish */
+#define PIKS_PREC_FP32_REFINED 1 /* FP64 defect correction: FP32 correction iterations between
+                                    FP64 residual iterations (see piks_set_precision) */
 
 /* para.in contents (src/parameters.F90:78-82), after defaults.  nx is the
  * image width before the fluid layers are added (src/parameters.F90:121). */
@@ -79,6 +80,9 @@ typedef struct piks_info {
     int64_t device_bytes;          /* device memory held by the context */
     int32_t coupled_walls;         /* walls whose extrapolation reads another wall */
     int32_t precision;
+    int64_t steps_fp32;            /* FP32 correction iterations run (PIKS_PREC_FP32_REFINED) */
+    int32_t dc_block;              /* current defect-correction block length (1: plain FP64) */
+    int32_t reserved;
 } piks_info;
 
 /* Set-up: replaces setupVelocityGrid + setupPhysicalGrid + setupFlow +
@@ -96,15 +100,28 @@ int piks_create(const piks_params* p, const uint8_t* raw_image, int64_t image_by
  * called from src/main.F90:100). */
 int piks_iterate(void* ctx, int32_t nsteps);
 
-/* Precision the next iterations run in, for a context created with
- * PIKS_PREC_FP32_REFINED (a PIKS_PREC_FP64 context accepts only FP64).  The
- * carried state (f, fw and the x exchange planes) is converted on the device;
- * moments, wall densities and the mass-flow check stay FP64 throughout.  The
- * refined solve runs FP32 iterations until the check's relative change
- * stalls, then switches to FP64 for the last iterations, so its converged
- * permeability is the FP64 solve's.  No counterpart in the reference, which
- * is FP64 only. */
+/* Precision mode of the next iterations, for a context created with
+ * PIKS_PREC_FP32_REFINED (a PIKS_PREC_FP64 context accepts only FP64).
+ * PIKS_PREC_FP32_REFINED is an FP64 defect correction of the reference's
+ * iteration, an affine map f <- T(f) (src/solver.F90:111-1940, the source
+ * being the pressure drop of src/solver.F90:1832-1862): piks_iterate runs
+ * blocks of m iterations, one FP64 iteration of the state S that yields the
+ * residual r = T(S) - S, then m - 1 FP32 iterations of the correction
+ * e <- A(e + r) with the linear part A (pressure drop 0); S + e + r then
+ * equals the plain iteration's state in exact arithmetic, so FP32 rounds
+ * only the correction and every check tracks the FP64 solve (within 1e-8 on
+ * the permeability).  m follows each piks_chk_converge (1 while the relative
+ * change per iteration is above 1e-2, then about sqrt(0.1 / change), at
+ * most 200; PIKS_DC_BLOCK in the environment fixes it).  Moments, wall
+ * densities and the mass-flow check stay FP64.  Switching to PIKS_PREC_FP64
+ * folds the correction into the FP64 state.  No counterpart in the
+ * reference, which is FP64 only. */
 int piks_set_precision(void* ctx, int32_t precision);
+
+/* Defect-correction block length of a PIKS_PREC_FP32_REFINED context: m >= 1
+ * fixes it from the next block on (1: plain FP64 iterations), m = 0 returns
+ * to the length that follows the checks.  No counterpart in the reference. */
+int piks_set_correction_block(void* ctx, int32_t m);
 
 /* Convergence check; replaces chkConverge (src/output.F90:60-164):
  * out[0] = mass flow, out[1] = apparent permeability, out[2] = error, and the
@@ -192,7 +209,7 @@ typedef struct piks_serve_result {
     int32_t status;            /* PIKS_REQ_* */
     int32_t error_code;        /* PIKS_E_* when FAILED */
     int32_t steps;             /* iterations run */
-    int32_t steps32;           /* of which in FP32 (PIKS_PREC_FP32_REFINED requests) */
+    int32_t steps32;           /* of which FP32 correction iterations (PIKS_PREC_FP32_REFINED) */
     int32_t converged;         /* 1: error <= eps was reached */
     int32_t reserved0;
     double  permeability;      /* last check's K (src/output.F90:155) */
@@ -205,8 +222,9 @@ typedef struct piks_serve_result {
 } piks_serve_result;
 
 int piks_serve_create(const piks_serve_params* sp, void** server_out);
-/* Queue one solve; p and the image as for piks_create.  refine_switch is the
- * FP32 -> FP64 switch level of FP32_REFINED requests (<= 0: 1e-6). */
+/* Queue one solve; p and the image as for piks_create.  refine_switch is
+ * ignored (kept for binary compatibility; FP32_REFINED requests run the
+ * defect correction of piks_set_precision). */
 int piks_serve_submit(void* server, const piks_params* p, const uint8_t* raw_image, int64_t image_bytes,
                       int32_t max_step, double eps, double refine_switch, int64_t* ticket_out);
 int piks_serve_poll(void* server, int64_t ticket, piks_serve_result* out);   /* no wait */

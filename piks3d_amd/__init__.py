@@ -54,6 +54,7 @@ class PiksInfo(ctypes.Structure):
         ("ds", ctypes.c_double), ("mu", ctypes.c_double), ("diff_flux", ctypes.c_double),
         ("mass", ctypes.c_double), ("device_bytes", ctypes.c_int64),
         ("coupled_walls", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("steps_fp32", ctypes.c_int64), ("dc_block", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -87,6 +88,7 @@ _API = {
                                    ctypes.POINTER(ctypes.c_void_p)]),
     "piks_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "piks_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "piks_set_correction_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "piks_chk_converge": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_mass_flow": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_get_macro": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),
@@ -174,12 +176,6 @@ def preprocess(params: Params, image):
     return labels, info
 
 
-def fp64_tail(max_step, chk):
-    """Iterations at the end of a refined solve's budget that always run in
-    FP64: the larger of three checks and a fifth of max_step."""
-    return max(3 * chk, max_step // 5)
-
-
 class Solver:
     """One DVM solve on one GPU (see module docstring)."""
 
@@ -222,6 +218,12 @@ class Solver:
     def set_precision(self, precision):
         code = {"fp64": PIKS_PREC_FP64, "fp32": PIKS_PREC_FP32_REFINED}.get(precision, precision)
         _check(lib().piks_set_precision(self._h, int(code)))
+        self.info = self.get_info()
+
+    def set_correction_block(self, m):
+        """Fix the defect-correction block length (m >= 1) of a refined
+        context, or return to the adaptive one (m = 0)."""
+        _check(lib().piks_set_correction_block(self._h, int(m)))
         self.info = self.get_info()
 
     def chk_converge(self):
@@ -267,19 +269,16 @@ class Solver:
         """The reference's main loop (src/main.F90:95-121): returns a dict with
         steps, permeability, mass, error and the check history.
 
-        A context created with precision 'fp32' runs FP32 iterations while
-        the check's error is above refine_switch, still falling, and the
-        iteration budget's FP64 tail (fp64_tail) has not begun, then switches
-        to FP64 for the remaining iterations; `steps32` in the result counts
-        the FP32 ones."""
+        A context created with precision 'fp32' runs the library's FP64
+        defect correction (FP32 correction iterations between FP64 residual
+        iterations, include/piks3d.h piks_set_precision); `steps32` in the
+        result counts its FP32 iterations.  refine_switch is accepted and
+        ignored."""
         p = self.params
         max_step = p.max_step if max_step is None else max_step
         chk = p.chk_converge_step if chk_step is None else chk_step
         eps = p.eps if eps is None else eps
-        tau = p.refine_switch if refine_switch is None else refine_switch
-        refined = self.info.precision == PIKS_PREC_FP32_REFINED
-        steps32 = 0
-        best, stalled = float("inf"), 0
+        start32 = self.get_info().steps_fp32
         error = 1.0
         perm = 0.0
         step = 0
@@ -289,34 +288,19 @@ class Solver:
             n = min(chk - step % chk, max_step - step)
             self.iterate(n)
             step += n
-            if refined:
-                steps32 += n
-            checked = step % chk == 0
-            if checked:
+            if step % chk == 0:
                 m, perm, error = self.chk_converge()
                 self.history.append((step, m, perm, error))
                 if callback:
                     callback(step, m, perm, error)
-            if refined:
-                # FP32 has converged as far as it can once the error is below
-                # the switch level or has not improved for three checks
-                if checked:
-                    if error < best:
-                        best, stalled = error, 0
-                    else:
-                        stalled += 1
-                    if error <= max(tau, eps) or stalled >= 3 or step >= max_step - fp64_tail(max_step, chk):
-                        self.set_precision("fp64")
-                        refined = False
-                        error = 1.0
-                continue
             if error <= eps:
                 if step < chk:
                     m, perm, error = self.chk_converge()
                     self.history.append((step, m, perm, error))
                 break
-        return dict(steps=step, steps32=steps32, permeability=perm, mass=self.mass, error=error,
-                    history=list(self.history))
+        self.info = self.get_info()
+        return dict(steps=step, steps32=int(self.info.steps_fp32 - start32), permeability=perm, mass=self.mass,
+                    error=error, history=list(self.history))
 
 
 class Server:
@@ -416,18 +400,14 @@ class MultiSolver(Solver):
 
     def solve(self, max_step=None, chk_step=None, eps=None, callback=None, refine_switch=None):
         """The reference's loop for every right-hand side at once: stops when
-        each has error <= eps (or at max_step).  In an FP32-refined context the
-        switch to FP64 happens when every right-hand side has reached the
-        switch criterion."""
+        each has error <= eps (or at max_step).  An FP32-refined context runs
+        the defect correction for all right-hand sides together, its block
+        length following the largest error."""
         p = self.params
         max_step = p.max_step if max_step is None else max_step
         chk = p.chk_converge_step if chk_step is None else chk_step
         eps = p.eps if eps is None else eps
-        tau = p.refine_switch if refine_switch is None else refine_switch
-        refined = self.info.precision == PIKS_PREC_FP32_REFINED
-        steps32 = 0
-        best = np.full(self.nrhs, np.inf)
-        stalled = np.zeros(self.nrhs, dtype=int)
+        start32 = self.get_info().steps_fp32
         error = np.ones(self.nrhs)
         perm = np.zeros(self.nrhs)
         step = 0
@@ -435,23 +415,14 @@ class MultiSolver(Solver):
             n = min(chk - step % chk, max_step - step)
             self.iterate(n)
             step += n
-            if refined:
-                steps32 += n
             if step % chk:
                 continue
             m, perm, error = self.chk_converge_all()
             self.history.append((step, m.copy(), perm.copy(), error.copy()))
             if callback:
                 callback(step, m, perm, error)
-            if refined:
-                improved = error < best
-                best = np.where(improved, error, best)
-                stalled = np.where(improved, 0, stalled + 1)
-                if np.all((error <= max(tau, eps)) | (stalled >= 3)) or step >= max_step - fp64_tail(max_step, chk):
-                    self.set_precision("fp64")
-                    refined = False
-                continue
             if np.all(error <= eps):
                 break
-        return dict(steps=step, steps32=steps32, kn=list(self.kn_values), permeability=perm, error=error,
-                    history=list(self.history))
+        self.info = self.get_info()
+        return dict(steps=step, steps32=int(self.info.steps_fp32 - start32), kn=list(self.kn_values),
+                    permeability=perm, error=error, history=list(self.history))

@@ -125,6 +125,29 @@ struct Ctx {
   uint32_t* tl_rec = nullptr;
   void* view_args[2] = {nullptr, nullptr};  // per precision, device copies for the sweep
   int32_t* pos[4] = {nullptr, nullptr, nullptr, nullptr};
+  // FP64 defect correction (PIKS_PREC_FP32_REFINED contexts).  The solve
+  // runs in blocks of dc_len iterations: one FP64 iteration of the state S
+  // that also yields the residual r = T(S) - S of the iteration map T (the
+  // swept slots' r comes from the sweep itself, the other slots' from
+  // ns_old, the copy taken before the iteration), then dc_len - 1 FP32
+  // iterations of the correction e <- A(e + r) with the linear part A of T
+  // (pressure drop 0), kept in s32 with the moments in mom_e.  In exact
+  // arithmetic S + e + r after j corrections is the plain iteration's state
+  // j + 1 iterations after the block began (T^k(S) = S + sum_{i<k} A^i r), so
+  // FP32 rounds only the correction; the state the host sees is S + s32
+  // (dc_pending) or S, folded into S by the next block or by ctx_dc_flush.
+  // dc_len follows the convergence check (ctx_dc_policy): 1 (plain FP64)
+  // while the per-iteration change is large, longer blocks as it falls.
+  State<float> r32;
+  double* ns_old = nullptr;          // [rhs][non-swept slot][velocity]
+  double *fw_old = nullptr, *bufe_old = nullptr, *bufw_old = nullptr;
+  Mom *mom_e = nullptr, *mom_old = nullptr, *mom_r = nullptr;
+  uint32_t* ns_list = nullptr;       // the non-swept slots of f (build_nonswept)
+  std::vector<uint32_t> ns_host;
+  int64_t n_ns = 0;
+  int dc_len = 1, dc_pos = 0, dc_fixed = 0;
+  bool dc_pending = false;
+  int64_t dc_inner = 0;              // FP32 correction iterations run
   // one iteration captured as a HIP graph, per precision (used by ctx_enqueue
   // when use_graph is set)
   bool use_graph = false;
@@ -160,8 +183,14 @@ double ctx_mass(const Ctx& c, const double* plane);
 void ctx_check(Ctx& c, int r, double m, double out[3]);
 // Set the right-hand sides before ctx_create_device: nrhs Knudsen numbers.
 int ctx_set_rhs(Ctx& c, int nrhs, const double* kn);
-// Switch the carried state's precision (synchronises the stream).
+// Switch between plain FP64 iterations and defect correction (refined
+// contexts only); a pending correction is folded into the FP64 state first.
 int ctx_set_phase(Ctx& c, int phase);
+// Fold a pending correction into the FP64 state (enqueued on the stream).
+int ctx_dc_flush(Ctx& c);
+// Block length of the defect correction after a check whose relative change
+// per iteration was err (the largest over the right-hand sides).
+void ctx_dc_policy(Ctx& c, double err);
 // Free everything (stream-ordered frees on a caller's stream do not block).
 void ctx_destroy(Ctx* c);
 

@@ -660,6 +660,31 @@ bool build_tile_schedule(Geometry& g, int TX, int TY, int TZ, int threads) {
   return true;
 }
 
+std::vector<uint32_t> build_nonswept(const Geometry& g, int threads) {
+  std::vector<std::vector<uint32_t>> per(8);
+  parallel_for(8, threads, [&](int o) {
+    const std::vector<int32_t>& pos = g.pos[ofam(o)];
+    std::vector<uint32_t>& out = per[(size_t)o];
+    for (int k = -1; k <= g.Nz + 2; ++k)
+      for (int j = -1; j <= g.Ny + 2; ++j)
+        for (int i = -1; i <= g.Nx + 2; ++i) {
+          const int64_t c = g.cell(i, j, k);
+          const int64_t p = pos[(size_t)c];
+          if (p == g.nstore) continue;
+          const int oi = osx(o) > 0 ? i - 1 : g.Nx - i;
+          const int oj = osy(o) > 0 ? j - 1 : g.Ny - j;
+          const int ok = osz(o) > 0 ? k - 1 : g.Nz - k;
+          const bool inside = i >= 1 && i <= g.Nx && j >= 1 && j <= g.Ny && k >= 1 && k <= g.Nz;
+          if (inside && oi >= 1 && oj >= 1 && ok >= 1 && g.lab[(size_t)c] == FLUID) continue;
+          out.push_back((uint32_t)p | (uint32_t)o << 28);
+        }
+    std::sort(out.begin(), out.end());
+  });
+  std::vector<uint32_t> all;
+  for (auto& v : per) all.insert(all.end(), v.begin(), v.end());
+  return all;
+}
+
 void build_wall_positions(Geometry& g, int threads) {
   const int64_t ns = g.nstore;
   // storage positions the wall kernels read and write

@@ -157,6 +157,12 @@ constexpr int tile_rec_xhalo(int ns, int q) { return 1 + ns + q; }
 constexpr int tile_rec_yhalo(int ns, int ty, int q) { return 1 + ns + 2 * ty + q; }
 constexpr int tile_rec_cell(int ns, int tx, int ty, int s) { return 1 + ns + 2 * ty + 2 * tx + s; }
 
+// The stored slots (octant o, storage index p) that octant o's sweep does not
+// update (walls, ghost planes, inflow layers, ...), as p | o << 28 sorted by
+// octant and index: the slots a defect-correction iteration updates outside
+// the sweep.  Needs the storage order.
+std::vector<uint32_t> build_nonswept(const Geometry& g, int threads);
+
 // Fill the tiled sweep schedule (tl_*); needs the storage order.  Returns
 // false when a tile shape does not fit the record (the schedule stays empty).
 bool build_tile_schedule(Geometry& g, int tx, int ty, int tz, int threads);

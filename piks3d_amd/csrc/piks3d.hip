@@ -173,6 +173,15 @@ struct SweepArgs {
   int32_t ntask;
   int32_t Nc8;
   uint32_t xcd_per;        // blocks per XCD (grid / 8) when renumbering, else 0
+  // defect correction (kernels with kDC != 0; same layout and windows as f):
+  // kDC = 1, the FP64 iteration of a block: the previous value of a swept
+  // slot is f + dc_e when dc_fold (the block before left a correction), and
+  // dc_r receives new - previous; kDC = 2, an FP32 correction iteration (f
+  // is the correction e): the previous value is e + dc_r, or dc_r alone in
+  // the block's first correction iteration (dc_first)
+  float* dc_r;
+  const float* dc_e;
+  int32_t dc_fold, dc_first;
 };
 
 // Everything a read of a switched wall slot needs (device memory).
@@ -288,6 +297,26 @@ __device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell
   return view_fix_body(va, cell, o, ll, rhs);
 }
 
+// Previous value of a swept slot (byte offset `self` in f's window) as the
+// update reads it; with defect correction (kDC, see SweepArgs) the stored
+// value plus the correction terms.
+template <typename T, int kDC>
+__device__ __forceinline__ double dc_previous(const SweepArgs<T>& a, __amdgpu_buffer_rsrc_t rs,
+                                              __amdgpu_buffer_rsrc_t rsr, __amdgpu_buffer_rsrc_t rse,
+                                              uint32_t voff, uint32_t self) {
+  const double v = buf_load<T>(rs, voff, self);
+  if constexpr (kDC == 1) {
+    // FP64 iteration of a block: fold the previous block's correction
+    const uint32_t s4 = self * 4u / (uint32_t)sizeof(T), v4 = voff * 4u / (uint32_t)sizeof(T);
+    return a.dc_fold ? v + (double)buf_load<float>(rse, v4, s4) : v;
+  } else if constexpr (kDC == 2) {
+    // correction iteration: e + r (e is f here), r alone the first time
+    const double r = buf_load<float>(rsr, voff, self);
+    return a.dc_first ? r : v + r;
+  }
+  return v;
+}
+
 // One hyperplane of the eight octant sweeps.  Every task (cell, octant) has
 // its upwind neighbours on the two previous hyperplanes (earlier launches), so
 // no wave waits on another.  A wave updates one task, all its velocities (64
@@ -299,7 +328,7 @@ __device__ __noinline__ ViewFix<T> view_fix(const ViewArgs<T>* va, uint32_t cell
 // Each update is src/solver.F90:206-214 (and analogues) with the same
 // operations in the same order (contraction off): signs enter as exact
 // negations, and a first-order axis contributes (c*0)*f like the reference.
-template <typename T, int kThreads, bool kRecip, int RB>
+template <typename T, int kThreads, bool kRecip, int RB, int kDC = 0>
 __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
   // the update runs in FP64 in both phases: FP32 arithmetic leaves a larger
   // error along the iteration's slowest mode than FP32 storage alone
@@ -321,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
   const int nr = a.nrhs - r0 < RB ? a.nrhs - r0 : RB;
   Mom m[RB];
   C mu[RB];
-  __amdgpu_buffer_rsrc_t rs[RB];
+  __amdgpu_buffer_rsrc_t rs[RB], rsr[RB], rse[RB];
 #pragma unroll
   for (int k = 0; k < RB; ++k) {
     const int rr = k < nr ? r0 + k : r0;
@@ -329,6 +358,13 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
     mu[k] = a.nrhs == 1 ? a.mu0 : ld_const(a.mu + rr);
     rs[k] = __builtin_amdgcn_make_buffer_rsrc(a.f + rr * a.rs_f + win.off, 0,
                                               (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
+    if constexpr (kDC != 0) {
+      rsr[k] = __builtin_amdgcn_make_buffer_rsrc(a.dc_r + rr * a.rs_f + win.off, 0, (int)(win.len * 4u), kBufferWord3);
+      rse[k] = __builtin_amdgcn_make_buffer_rsrc((float*)a.dc_e + rr * a.rs_f + win.off, 0, (int)(win.len * 4u),
+                                                 kBufferWord3);
+    } else {
+      rsr[k] = rse[k] = rs[k];  // not read without defect correction
+    }
   }
   const int sx = osx(o), sy = osy(o), sz = osz(o);
   const int Nc8 = a.Nc8;
@@ -338,10 +374,11 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
     const bool lact = l < Nc8;
     const int ll = lact ? l : 0;
     const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
-    T fp[RB], nb[RB][6];
+    T nb[RB][6];
+    C fp[RB];
 #pragma unroll
     for (int k = 0; k < RB; ++k) {
-      fp[k] = buf_load<T>(rs[k], voff, self);
+      fp[k] = dc_previous<T, kDC>(a, rs[k], rsr[k], rse[k], voff, self);
 #pragma unroll
       for (int q = 0; q < 6; ++q) nb[k][q] = buf_load<T>(rs[k], voff, (uint32_t)r.w[1 + q] * (uint32_t)sizeof(T));
     }
@@ -376,7 +413,10 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
         den = den + czl * (fz ? d1 : c0);
         fnew = num / den;
       }
-      if (lact) buf_store((T)fnew, rs[k], voff, self);
+      if (lact) {
+        buf_store((T)fnew, rs[k], voff, self);
+        if constexpr (kDC == 1) buf_store((float)(fnew - fp[k]), rsr[k], (uint32_t)ll * 4u, self * 4u / (uint32_t)sizeof(T));
+      }
       if (a.part) {
         // moments are summed in FP64 in both phases
         const double fv = lact ? (double)(T)fnew : 0.0;
@@ -401,7 +441,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
 // in one pass of the wave (without the pass loop the FP64 kernel needs 52
 // VGPRs instead of 65: 8 waves per SIMD instead of 7); k_sweep1_passes loops
 // over passes of 64 velocities for larger grids.
-template <typename T, bool kRecip, bool kOnePass>
+template <typename T, bool kRecip, bool kOnePass, int kDC = 0>
 __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
   // the padding waves of the grid (t >= ntask) run the last task without
   // storing anything, so that no load waits behind the bounds test
@@ -418,6 +458,11 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
   const SweepCoef cf = ld_const(a.coef + (flg & 7u));
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(a.f + win.off, 0, (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
+  __amdgpu_buffer_rsrc_t rsr = rs, rse = rs;
+  if constexpr (kDC != 0) {
+    rsr = __builtin_amdgcn_make_buffer_rsrc(a.dc_r + win.off, 0, (int)(win.len * 4u), kBufferWord3);
+    rse = __builtin_amdgcn_make_buffer_rsrc((float*)a.dc_e + win.off, 0, (int)(win.len * 4u), kBufferWord3);
+  }
   const int Nc8 = a.Nc8;
   // signed moments: cx*(-u) is -(cx*u) exactly, and t - y is t + (-y)
   const double mx = osx((int)o) > 0 ? m.x : -m.x;
@@ -429,7 +474,7 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
     const bool lact = l < Nc8;
     const int ll = lact ? l : 0;
     const uint32_t voff = (uint32_t)ll * (uint32_t)sizeof(T);
-    const double fp = buf_load<T>(rs, voff, self);
+    const double fp = dc_previous<T, kDC>(a, rs, rsr, rse, voff, self);
     double nb[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) nb[q] = buf_load<T>(rs, voff, (uint32_t)r.w[1 + q] * (uint32_t)sizeof(T));
@@ -467,7 +512,10 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
       den = den + czl * cf.dz;
       fnew = num / den;
     }
-    if (lact && live) buf_store((T)fnew, rs, voff, self);
+    if (lact && live) {
+      buf_store((T)fnew, rs, voff, self);
+      if constexpr (kDC == 1) buf_store((float)(fnew - fp), rsr, (uint32_t)ll * 4u, self * 4u / (uint32_t)sizeof(T));
+    }
     if (a.part && live) {
       const double fv = lact ? (double)(T)fnew : 0.0;
       const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv);
@@ -484,12 +532,12 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
   }
 }
 
-template <typename T, bool kRecip>
-__global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) { sweep1_task<T, kRecip, true>(a); }
+template <typename T, bool kRecip, int kDC = 0>
+__global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) { sweep1_task<T, kRecip, true, kDC>(a); }
 
 // the same task for grids of more than 64 velocities per octant: passes of 64
-template <typename T, bool kRecip>
-__global__ __launch_bounds__(64) void k_sweep1_passes(SweepArgs<T> a) { sweep1_task<T, kRecip, false>(a); }
+template <typename T, bool kRecip, int kDC = 0>
+__global__ __launch_bounds__(64) void k_sweep1_passes(SweepArgs<T> a) { sweep1_task<T, kRecip, false, kDC>(a); }
 
 // ------------------------------------------------------------------ tiled sweep
 
@@ -1227,27 +1275,86 @@ __global__ void k_moments_serial(const T* f, const int32_t* cells, int64_t n, co
   }
 }
 
-__global__ void k_gather_plane(const Mom* mom, double* out, Dims d, int column) {
+// Ux on the plane x = column; with a pending defect correction (mom_e not
+// null) the state's Ux is mom + mom_e.
+__global__ void k_gather_plane(const Mom* mom, const Mom* mom_e, double* out, Dims d, int column) {
   const int64_t n = (int64_t)d.Ny * d.Nz;
   mom += blockIdx.y * d.Ntotal;
+  if (mom_e) mom_e += blockIdx.y * d.Ntotal;
   out += blockIdx.y * n;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int j = 1 + (int)(t % d.Ny), k = 1 + (int)(t / d.Ny);
-    out[t] = mom[(int64_t)(column + 1) + (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt].x;
+    const int64_t cl = (int64_t)(column + 1) + (int64_t)(j + 1) * d.Nxt + (int64_t)(k + 1) * d.Nxyt;
+    out[t] = mom_e ? mom[cl].x + mom_e[cl].x : mom[cl].x;
   }
 }
 
 // ------------------------------------------------------------------ context
 
-template <typename T>
-__global__ void k_convert(T* dst, const double* src, size_t n) {
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-    dst[t] = (T)src[t];
+// ------------------------------------------------------------------ defect correction
+
+// Element-wise steps of a defect-correction block (see Ctx) over arrays kept
+// whole (fw, the exchange planes, the moments: every right-hand side at once).
+// Before the block's FP64 iteration: s += e when a correction is pending,
+// and old = s.
+template <typename TS, typename TE>
+__global__ void k_dc_begin_arr(TS* s, const TE* e, TS* old, size_t n, int fold) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const TS v = fold ? s[t] + (TS)e[t] : s[t];
+    s[t] = v;
+    old[t] = v;
+  }
 }
-__global__ void k_widen(double* dst, const float* src, size_t n) {
+// After it: r = s - old.
+template <typename TS, typename TR>
+__global__ void k_dc_resid_arr(const TS* s, const TS* old, TR* r, size_t n) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-    dst[t] = (double)src[t];
+    r[t] = (TR)(s[t] - old[t]);
+}
+// Before each correction iteration: e = r (the block's first) or e + r.
+template <typename TE, typename TR>
+__global__ void k_dc_add_arr(TE* e, const TR* r, size_t n, int first) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    e[t] = first ? (TE)r[t] : e[t] + (TE)r[t];
+}
+// Fold a pending correction into the state: s += e.
+template <typename TS, typename TE>
+__global__ void k_dc_flush_arr(TS* s, const TE* e, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    s[t] = s[t] + (TS)e[t];
+}
+
+// The same steps over the slots of f that no sweep updates (list entries
+// storage index | octant << 28), one wave per slot and one velocity per lane;
+// old holds [right-hand side][list entry][velocity].  The swept slots take
+// these steps inside the sweep kernels (dc_previous, kDC).
+// mode 0: begin (fold, copy), 1: residual, 2: add (first = e = r)
+__global__ __launch_bounds__(256) void k_dc_slots(double* s, float* e, double* old, float* r,
+                                                  const uint32_t* list, int64_t n, Dims d, int mode, int flag) {
+  const int rhs = blockIdx.y;
+  s += rhs * d.rs_f;
+  e += rhs * d.rs_f;
+  r += rhs * d.rs_f;
+  old += (size_t)rhs * n * d.Nc8;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpb = blockDim.x / kWave;
+  for (int64_t idx = (int64_t)blockIdx.x * wpb + wave_in_block(); idx < n; idx += (int64_t)gridDim.x * wpb) {
+    const uint32_t w = list[idx];
+    const size_t at = (size_t)(w >> 28) * d.ostride + (size_t)(w & 0x0fffffffu) * d.Nc8;
+    double* oo = old + (size_t)idx * d.Nc8;
+    for (int l = lane; l < d.Nc8; l += kWave) {
+      if (mode == 0) {
+        const double v = flag ? s[at + l] + (double)e[at + l] : s[at + l];
+        s[at + l] = v;
+        oo[l] = v;
+      } else if (mode == 1) {
+        r[at + l] = (float)(s[at + l] - oo[l]);
+      } else {
+        e[at + l] = flag ? r[at + l] : e[at + l] + r[at + l];
+      }
+    }
+  }
 }
 
 template <typename T>
@@ -1347,16 +1454,45 @@ void (*tile_kernel(int tx, int ty))(TileArgs<T>) {
   return nullptr;
 }
 
-// Enqueue nsteps iterations on the context's stream.  With `timed`, four
-// events per step (at most kMaxTimedSteps) bracket the sweep, boundary and
-// moment phases; the caller synchronises and reads them.
+// What one call of enqueue_t iterates: the moments it reads and writes, the
+// inlet/outlet pressure drop, and the defect-correction role of its sweep
+// (SweepArgs: dc 0 plain, 1 a block's FP64 iteration, 2 a correction
+// iteration, with the fold / first flags).
+struct IterMode {
+  Mom* mom;
+  double pd;
+  int dc, fold, first;
+};
+
+// Single-solve sweep kernels of a defect-correction role.
+template <typename T, int kDC>
+SweepKernel<T> sweep_kernel_dc(const Ctx& c, int rb) {
+  if (c.nrhs == 1) {
+    const bool recip = c.sweep_recip && !c.exact;
+    if (c.d.Nc8 <= kWave) return recip ? k_sweep1<T, true, kDC> : k_sweep1<T, false, kDC>;
+    return recip ? k_sweep1_passes<T, true, kDC> : k_sweep1_passes<T, false, kDC>;
+  }
+  switch (rb) {
+    case 2: return k_sweep<T, 64, false, 2, kDC>;
+    case 4: return k_sweep<T, 64, false, 4, kDC>;
+    default: return k_sweep<T, 64, false, 1, kDC>;
+  }
+}
+
+// Enqueue nsteps iterations on the context's stream.  With ev0 >= 0, four
+// events per step (steps ev0, ev0 + 1, ..., below kMaxTimedSteps) bracket the
+// sweep, boundary and moment phases; the caller synchronises and reads them.
 template <typename T>
-int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
+int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
   const Geometry& g = c.g;
   const Dims d = c.d;
   SweepArgs<T> sa{};
   sa.f = st.f;
-  sa.mom = c.mom;
+  sa.mom = im.mom;
+  sa.dc_r = c.r32.f;
+  sa.dc_e = c.s32.f;
+  sa.dc_fold = im.fold;
+  sa.dc_first = im.first;
   sa.part = c.exact ? nullptr : c.part;
   sa.cx = c.dcx; sa.cy = c.dcy; sa.cz = c.dcz; sa.w = c.dw;
   sa.mu = c.d_mu;
@@ -1372,17 +1508,22 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
   const int rb = c.nrhs >= 4 ? 4 : c.nrhs >= 2 ? 2 : 1;
   const unsigned ny_sweep = (unsigned)((c.nrhs + rb - 1) / rb), R = (unsigned)c.nrhs;
   const int wpb = 4;  // waves per block for one-wave-per-item kernels
-  const int timed = timed_run ? std::min(nsteps, kMaxTimedSteps) : 0;
-  while ((int)c.ev.size() < 4 * timed) {
+  const int timed = ev0 >= 0 ? std::max(0, std::min(nsteps, kMaxTimedSteps - ev0)) : 0;
+  while ((int)c.ev.size() < 4 * (ev0 + timed)) {
     hipEvent_t e;
     HIP_TRY(hipEventCreate(&e));
     c.ev.push_back(e);
+  }
+  SweepKernel<T> kdc = nullptr;
+  if (im.dc != 0) {
+    if constexpr (sizeof(T) == sizeof(double)) kdc = sweep_kernel_dc<T, 1>(c, rb);
+    else kdc = sweep_kernel_dc<T, 2>(c, rb);
   }
   TileArgs<T> ta{};
   void (*ktile)(TileArgs<T>) = nullptr;
   if (c.sweep_tiles) {
     ta.f = st.f;
-    ta.mom = c.mom;
+    ta.mom = im.mom;
     ta.part = c.exact ? nullptr : c.part;
     ta.cx = c.dcx; ta.cy = c.dcy; ta.cz = c.dcz; ta.w = c.dw;
     ta.mu = c.d_mu;
@@ -1400,7 +1541,8 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
   }
   for (int step = 0; step < nsteps; ++step) {
     const bool tm = step < timed;
-    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 0], c.stream));
+    const int ev = 4 * (ev0 + step);
+    if (tm) HIP_TRY(hipEventRecord(c.ev[ev + 0], c.stream));
     if (c.sweep_tiles) {
       const size_t per_tile = (size_t)(g.tl_tz + 1) * Geometry::kTileRecDw;
       for (int dg = 0; dg < g.tl_ndiag; ++dg) {
@@ -1421,6 +1563,10 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
       const int64_t wpg = c.sweep_block / kWave;
       const int64_t blocks = ((sa.ntask + wpg - 1) / wpg + 7) / 8 * 8;
       sa.xcd_per = c.xcd_remap ? (uint32_t)(blocks / 8) : 0u;
+      if (kdc) {
+        hipLaunchKernelGGL(kdc, dim3((unsigned)blocks, c.nrhs == 1 ? 1u : ny_sweep), dim3(64), 0, c.stream, sa);
+        continue;
+      }
       if (c.nrhs == 1 && c.sweep_block == 64 && !c.sweep_generic) {
         const bool recip = c.sweep_recip && !c.exact;
         SweepKernel<T> k1 = d.Nc8 <= kWave ? (recip ? k_sweep1<T, true> : k_sweep1<T, false>)
@@ -1432,7 +1578,7 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
                          dim3((unsigned)blocks, ny_sweep),
                          dim3(c.sweep_block), 0, c.stream, sa);
     }
-    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 1], c.stream));
+    if (tm) HIP_TRY(hipEventRecord(c.ev[ev + 1], c.stream));
     if (!g.commit_walls.empty())
       hipLaunchKernelGGL(k_commit<T>, dim3(grid_for((int64_t)g.commit_walls.size(), wpb), R),
                          dim3(wpb * kWave), 0, c.stream, st.f, (const T*)st.fw, c.commit_walls,
@@ -1460,37 +1606,108 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, bool timed_run) {
                          c.stream, wa);
     }
     hipLaunchKernelGGL(k_inout<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Ny * g.Nz * 4, wpb) : grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256), R), dim3(256),
-                       0, c.stream, st.f, (const double*)c.dw, c.prm.press_drop, d);
+                       0, c.stream, st.f, (const double*)c.dw, im.pd, d);
     hipLaunchKernelGGL(k_sym_y<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Nz, wpb) : grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256), R), dim3(256), 0,
                        c.stream, st.f, d);
     hipLaunchKernelGGL(k_sym_z<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Ny, wpb) : grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256), R), dim3(256), 0,
                        c.stream, st.f, d);
-    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 2], c.stream));
+    if (tm) HIP_TRY(hipEventRecord(c.ev[ev + 2], c.stream));
     if (c.exact) {
       const int64_t n = (int64_t)g.fluid_interior.size() + (int64_t)g.fluid_boundary.size();
       hipLaunchKernelGGL(k_moments_serial<T>, dim3(grid_for(n, 64), R), dim3(64), 0, c.stream,
-                         (const T*)st.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, c.mom);
+                         (const T*)st.f, c.fluid_all, n, c.dcx, c.dcy, c.dcz, d, im.mom);
     } else {
       const int64_t ni = (int64_t)g.fluid_interior.size(), nbd = (int64_t)g.fluid_boundary.size();
       if (ni > 0)
         hipLaunchKernelGGL(k_moments_partials, dim3(grid_for(ni, 256), R), dim3(256), 0, c.stream,
-                           (const double*)c.part, c.fluid_int, ni, c.mom, g.Ntotal);
+                           (const double*)c.part, c.fluid_int, ni, im.mom, g.Ntotal);
       if (nbd > 0)
         hipLaunchKernelGGL(k_moments_direct<T>, dim3((unsigned)((nbd + wpb - 1) / wpb), R),
                            dim3(wpb * kWave), 0, c.stream, (const T*)st.f, c.fluid_bnd, nbd,
-                           c.dcx, c.dcy, c.dcz, d, c.mom);
+                           c.dcx, c.dcy, c.dcz, d, im.mom);
     }
-    if (tm) HIP_TRY(hipEventRecord(c.ev[4 * step + 3], c.stream));
+    if (tm) HIP_TRY(hipEventRecord(c.ev[ev + 3], c.stream));
   }
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// Launch shape of the element-wise defect-correction kernels.
+dim3 dc_grid(size_t n) { return dim3((unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 16384))); }
+
+// The element-wise part of a defect-correction step over the arrays kept
+// whole and the non-swept slots of f (see k_dc_begin_arr ... k_dc_slots).
+int dc_arrays(Ctx& c, int mode, int flag) {
+  const size_t R = (size_t)c.nrhs;
+  const size_t nfw = R * c.n_fw, nbuf = R * c.n_buf, nmom = R * (size_t)c.g.Ntotal * 4;
+  double* mom = reinterpret_cast<double*>(c.mom);
+  double* mom_e = reinterpret_cast<double*>(c.mom_e);
+  double* mom_old = reinterpret_cast<double*>(c.mom_old);
+  double* mom_r = reinterpret_cast<double*>(c.mom_r);
+  hipStream_t s = c.stream;
+  if (mode == 0) {
+    hipLaunchKernelGGL((k_dc_begin_arr<double, float>), dc_grid(nfw), dim3(256), 0, s, c.s64.fw, (const float*)c.s32.fw, c.fw_old, nfw, flag);
+    hipLaunchKernelGGL((k_dc_begin_arr<double, float>), dc_grid(nbuf), dim3(256), 0, s, c.s64.buf_e, (const float*)c.s32.buf_e, c.bufe_old, nbuf, flag);
+    hipLaunchKernelGGL((k_dc_begin_arr<double, float>), dc_grid(nbuf), dim3(256), 0, s, c.s64.buf_w, (const float*)c.s32.buf_w, c.bufw_old, nbuf, flag);
+    hipLaunchKernelGGL((k_dc_begin_arr<double, double>), dc_grid(nmom), dim3(256), 0, s, mom, (const double*)mom_e, mom_old, nmom, flag);
+  } else if (mode == 1) {
+    hipLaunchKernelGGL((k_dc_resid_arr<double, float>), dc_grid(nfw), dim3(256), 0, s, (const double*)c.s64.fw, (const double*)c.fw_old, c.r32.fw, nfw);
+    hipLaunchKernelGGL((k_dc_resid_arr<double, float>), dc_grid(nbuf), dim3(256), 0, s, (const double*)c.s64.buf_e, (const double*)c.bufe_old, c.r32.buf_e, nbuf);
+    hipLaunchKernelGGL((k_dc_resid_arr<double, float>), dc_grid(nbuf), dim3(256), 0, s, (const double*)c.s64.buf_w, (const double*)c.bufw_old, c.r32.buf_w, nbuf);
+    hipLaunchKernelGGL((k_dc_resid_arr<double, double>), dc_grid(nmom), dim3(256), 0, s, (const double*)mom, (const double*)mom_old, mom_r, nmom);
+  } else {
+    hipLaunchKernelGGL((k_dc_add_arr<float, float>), dc_grid(nfw), dim3(256), 0, s, c.s32.fw, (const float*)c.r32.fw, nfw, flag);
+    hipLaunchKernelGGL((k_dc_add_arr<float, float>), dc_grid(nbuf), dim3(256), 0, s, c.s32.buf_e, (const float*)c.r32.buf_e, nbuf, flag);
+    hipLaunchKernelGGL((k_dc_add_arr<float, float>), dc_grid(nbuf), dim3(256), 0, s, c.s32.buf_w, (const float*)c.r32.buf_w, nbuf, flag);
+    hipLaunchKernelGGL((k_dc_add_arr<double, double>), dc_grid(nmom), dim3(256), 0, s, mom_e, (const double*)mom_r, nmom, flag);
+  }
+  if (c.n_ns > 0)
+    hipLaunchKernelGGL(k_dc_slots, dim3(grid_for(c.n_ns, 4), (unsigned)c.nrhs), dim3(256), 0, s, c.s64.f, c.s32.f,
+                       c.ns_old, c.r32.f, (const uint32_t*)c.ns_list, c.n_ns, c.d, mode, flag);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// nsteps iterations of a refined context (defect correction, see Ctx),
+// timed from event step ev0 on (ev0 < 0: untimed).
+int dc_iterate(Ctx& c, int nsteps, int ev0) {
+  const IterMode plain{c.mom, c.prm.press_drop, 0, 0, 0};
+  for (int step = 0; step < nsteps; ++step) {
+    const int ev = ev0 >= 0 ? ev0 + step : -1;
+    if (c.dc_len <= 1) {
+      // plain FP64 iterations
+      if (int rc = ctx_dc_flush(c)) return rc;
+      if (int rc = enqueue_t<double>(c, c.s64, 1, ev, plain)) return rc;
+      continue;
+    }
+    if (c.dc_pos == 0 || c.dc_pos >= c.dc_len) {
+      // a block's FP64 iteration: fold the previous block's correction, keep
+      // the state, iterate, take the residual
+      const int fold = c.dc_pending ? 1 : 0;
+      if (int rc = dc_arrays(c, 0, fold)) return rc;
+      if (int rc = enqueue_t<double>(c, c.s64, 1, ev, IterMode{c.mom, c.prm.press_drop, 1, fold, 0})) return rc;
+      if (int rc = dc_arrays(c, 1, 0)) return rc;
+      c.dc_pending = false;
+      c.dc_pos = 1;
+      continue;
+    }
+    // a correction iteration: e <- A(e + r), pressure drop 0
+    const int first = c.dc_pos == 1 ? 1 : 0;
+    if (int rc = dc_arrays(c, 2, first)) return rc;
+    if (int rc = enqueue_t<float>(c, c.s32, 1, ev, IterMode{c.mom_e, 0.0, 2, 0, first})) return rc;
+    c.dc_pending = true;
+    ++c.dc_inner;
+    if (++c.dc_pos >= c.dc_len) c.dc_pos = 0;
+  }
   return 0;
 }
 
 // Timed synchronous iterations (piks_iterate).
 int iterate(Ctx& c, int nsteps) {
   HIP_TRY(hipSetDevice(c.device));
-  int rc = c.phase == PIKS_PREC_FP32_REFINED ? enqueue_t<float>(c, c.s32, nsteps, true)
-                                              : enqueue_t<double>(c, c.s64, nsteps, true);
+  int rc = c.phase == PIKS_PREC_FP32_REFINED
+               ? dc_iterate(c, nsteps, 0)
+               : enqueue_t<double>(c, c.s64, nsteps, 0, IterMode{c.mom, c.prm.press_drop, 0, 0, 0});
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c.stream));
   const int timed = std::min(nsteps, kMaxTimedSteps);
@@ -1554,6 +1771,11 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
     c->tile_tx = tx; c->tile_ty = ty; c->tile_tz = tz;
   }
   if (const char* b = std::getenv("PIKS_GRAPH")) c->use_graph = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_DC_BLOCK")) {
+    const int v = std::atoi(b);
+    if (v < 1 || v > 100000) return fail(PIKS_E_ARG, "PIKS_DC_BLOCK must be 1..100000");
+    c->dc_fixed = v;
+  }
   if (const char* b = std::getenv("PIKS_SWEEP_RECIP")) c->sweep_recip = std::atoi(b) != 0;
   if (const char* b = std::getenv("PIKS_SWEEP_GENERIC")) c->sweep_generic = std::atoi(b) != 0;
   c->mu = std::sqrt((std::atan(1.0) * 4.0) / 2.0) / p->kn;   // src/flow.F90:90
@@ -1566,7 +1788,12 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   c->d.rs_snap = (int64_t)c->n_snap;
   // the tiled sweep serves grids of at most 64 velocities per octant and the
   // generic kernel is a forced alternative
-  if (c->d.Nc8 > kWave || c->sweep_generic) c->sweep_tiles = false;
+  if (c->d.Nc8 > kWave || c->sweep_generic || p->precision == PIKS_PREC_FP32_REFINED) c->sweep_tiles = false;
+  if (p->precision == PIKS_PREC_FP32_REFINED) {
+    c->ns_host = build_nonswept(c->g, host_threads());
+    c->n_ns = (int64_t)c->ns_host.size();
+    c->dc_len = c->dc_fixed > 0 ? c->dc_fixed : 1;
+  }
   if (c->sweep_tiles) {
     Geometry& gt = c->g;
     if (!build_tile_schedule(gt, c->tile_tx, c->tile_ty, c->tile_tz, host_threads()))
@@ -1608,7 +1835,12 @@ size_t ctx_device_bytes(const Ctx& c) {
   const bool refined = c.prm.precision == PIKS_PREC_FP32_REFINED;
   const size_t R = (size_t)c.nrhs;
   const size_t state64 = R * (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(double);
-  const size_t state32 = refined ? R * (c.n_f + c.n_fw + 2 * c.n_buf + c.n_snap) * sizeof(float) : 0;
+  // refined: the FP32 correction (with the snapshot) and residual, the FP64
+  // copies of the non-swept slots and small arrays, three moment arrays
+  const size_t state32 = refined ? R * (2 * (c.n_f + c.n_fw + 2 * c.n_buf) + c.n_snap) * sizeof(float) +
+                                       R * ((size_t)c.n_ns * c.d.Nc8 + c.n_fw + 2 * c.n_buf) * sizeof(double) +
+                                       R * (size_t)g.Ntotal * 3 * sizeof(Mom) + (size_t)c.n_ns * 4
+                                 : 0;
   const size_t per_cell = (size_t)g.Ntotal * (R * (sizeof(Mom) + 32 * sizeof(double)) + 1 + 4 + 4 * 4);
   const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.wall_pos.size() * 4 +
                        g.uncoupled.size() * 4 +
@@ -1646,7 +1878,14 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
   if (refined) {
     if ((rc = dalloc(c, &c.s32.f, R * c.n_f)) || (rc = dalloc(c, &c.s32.fw, R * c.n_fw)) ||
         (rc = dalloc(c, &c.s32.buf_e, R * c.n_buf)) || (rc = dalloc(c, &c.s32.buf_w, R * c.n_buf)) ||
-        (rc = dalloc(c, &c.s32.snap, R * c.n_snap)))
+        (rc = dalloc(c, &c.s32.snap, R * c.n_snap)) || (rc = dalloc(c, &c.r32.f, R * c.n_f)) ||
+        (rc = dalloc(c, &c.r32.fw, R * c.n_fw)) || (rc = dalloc(c, &c.r32.buf_e, R * c.n_buf)) ||
+        (rc = dalloc(c, &c.r32.buf_w, R * c.n_buf)) ||
+        (rc = dalloc(c, &c.ns_old, R * (size_t)std::max<int64_t>(c.n_ns, 1) * c.d.Nc8)) ||
+        (rc = dalloc(c, &c.fw_old, R * c.n_fw)) || (rc = dalloc(c, &c.bufe_old, R * c.n_buf)) ||
+        (rc = dalloc(c, &c.bufw_old, R * c.n_buf)) || (rc = dalloc(c, &c.mom_e, R * (size_t)g.Ntotal)) ||
+        (rc = dalloc(c, &c.mom_old, R * (size_t)g.Ntotal)) || (rc = dalloc(c, &c.mom_r, R * (size_t)g.Ntotal)) ||
+        (rc = upload(c, &c.ns_list, c.ns_host)))
       return rc;
     c.phase = PIKS_PREC_FP32_REFINED;
   }
@@ -1728,14 +1967,14 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
   return 0;
 }
 
-// Capture one iteration of the current precision as a graph.  The captured
-// work has no memset or allocation node: only kernel launches.
+// Capture one plain FP64 iteration as a graph.  The captured work has no
+// memset or allocation node: only kernel launches.
 static int capture_graph(Ctx& c) {
-  const int k = c.phase == PIKS_PREC_FP32_REFINED ? 1 : 0;
+  const int k = 0;
   if (c.graph[k]) return 0;
   hipGraph_t gr = nullptr;
   HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  const int rc = k ? enqueue_t<float>(c, c.s32, 1, false) : enqueue_t<double>(c, c.s64, 1, false);
+  const int rc = enqueue_t<double>(c, c.s64, 1, -1, IterMode{c.mom, c.prm.press_drop, 0, 0, 0});
   const hipError_t e = hipStreamEndCapture(c.stream, &gr);
   if (rc) {
     if (gr) (void)hipGraphDestroy(gr);
@@ -1750,21 +1989,65 @@ static int capture_graph(Ctx& c) {
 
 int ctx_enqueue(Ctx& c, int nsteps) {
   HIP_TRY(hipSetDevice(c.device));
+  if (c.phase == PIKS_PREC_FP32_REFINED) return dc_iterate(c, nsteps, -1);
   if (c.use_graph) {
     if (int rc = capture_graph(c)) return rc;
-    const int k = c.phase == PIKS_PREC_FP32_REFINED ? 1 : 0;
-    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c.graph[k], c.stream));
+    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c.graph[0], c.stream));
     return 0;
   }
-  return c.phase == PIKS_PREC_FP32_REFINED ? enqueue_t<float>(c, c.s32, nsteps, false)
-                                            : enqueue_t<double>(c, c.s64, nsteps, false);
+  return enqueue_t<double>(c, c.s64, nsteps, -1, IterMode{c.mom, c.prm.press_drop, 0, 0, 0});
+}
+
+int ctx_dc_flush(Ctx& c) {
+  if (!c.dc_pending) {
+    c.dc_pos = 0;
+    return 0;
+  }
+  HIP_TRY(hipSetDevice(c.device));
+  const size_t R = (size_t)c.nrhs;
+  hipStream_t s = c.stream;
+  const size_t nf = R * c.n_f, nfw = R * c.n_fw, nbuf = R * c.n_buf, nmom = R * (size_t)c.g.Ntotal * 4;
+  hipLaunchKernelGGL((k_dc_flush_arr<double, float>), dc_grid(nf), dim3(256), 0, s, c.s64.f, (const float*)c.s32.f, nf);
+  hipLaunchKernelGGL((k_dc_flush_arr<double, float>), dc_grid(nfw), dim3(256), 0, s, c.s64.fw, (const float*)c.s32.fw, nfw);
+  hipLaunchKernelGGL((k_dc_flush_arr<double, float>), dc_grid(nbuf), dim3(256), 0, s, c.s64.buf_e, (const float*)c.s32.buf_e, nbuf);
+  hipLaunchKernelGGL((k_dc_flush_arr<double, float>), dc_grid(nbuf), dim3(256), 0, s, c.s64.buf_w, (const float*)c.s32.buf_w, nbuf);
+  hipLaunchKernelGGL((k_dc_flush_arr<double, double>), dc_grid(nmom), dim3(256), 0, s, reinterpret_cast<double*>(c.mom),
+                     (const double*)reinterpret_cast<double*>(c.mom_e), nmom);
+  HIP_TRY(hipGetLastError());
+  c.dc_pending = false;
+  c.dc_pos = 0;
+  return 0;
+}
+
+void ctx_dc_policy(Ctx& c, double err) {
+  if (c.phase != PIKS_PREC_FP32_REFINED) return;
+  if (c.dc_fixed > 0) {
+    c.dc_len = c.dc_fixed;
+    return;
+  }
+  // FP32 rounds a block's correction e (about m times the per-iteration
+  // change err, relative to the state) once per correction iteration, so a
+  // block adds a relative error of at most about eps32 * m^2 * err; m is
+  // sqrt(0.1 / err) (at most ~6e-9 per block; measured errors on the
+  // benchmark images stay two orders of magnitude below this bound), in
+  // whole check intervals, at most 200.  Above err = 1e-2 (the first checks
+  // of a solve) every iteration is FP64.
+  if (!(err <= 1e-2)) {
+    c.dc_len = 1;
+    return;
+  }
+  const int chk = std::max(1, c.prm.chk_converge_step);
+  int m = (int)std::floor(std::sqrt(0.1 / std::max(err, 1e-300)));
+  m = std::min(200, std::max(m, 2));
+  if (m >= chk) m = m / chk * chk;
+  c.dc_len = m;
 }
 
 int ctx_enqueue_plane(Ctx& c, double* host_plane) {
   const Geometry& g = c.g;
   HIP_TRY(hipSetDevice(c.device));
   hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)g.Ny * g.Nz, 256), (unsigned)c.nrhs), dim3(256), 0, c.stream,
-                     (const Mom*)c.mom, c.plane, c.d, 2);
+                     (const Mom*)c.mom, c.dc_pending ? (const Mom*)c.mom_e : nullptr, c.plane, c.d, 2);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(host_plane, c.plane, (size_t)g.Ny * g.Nz * c.nrhs * sizeof(double),
                          hipMemcpyDeviceToHost, c.stream));
@@ -1816,22 +2099,10 @@ int ctx_set_rhs(Ctx& c, int nrhs, const double* kn) {
 
 int ctx_set_phase(Ctx& c, int phase) {
   if (phase == c.phase) return 0;
-  if (phase == PIKS_PREC_FP32_REFINED && !c.s32.f)
-    return fail(PIKS_E_STATE, "context was created without FP32 storage");
+  if (phase == PIKS_PREC_FP32_REFINED && !c.r32.f)
+    return fail(PIKS_E_STATE, "context was created without FP32 correction storage");
   HIP_TRY(hipSetDevice(c.device));
-  const int blocks = 4096;
-  if (phase == PIKS_PREC_FP64) {
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.f, (const float*)c.s32.f, c.n_f * c.nrhs);
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.fw, (const float*)c.s32.fw, c.n_fw * c.nrhs);
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_e, (const float*)c.s32.buf_e, c.n_buf * c.nrhs);
-    hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, c.stream, c.s64.buf_w, (const float*)c.s32.buf_w, c.n_buf * c.nrhs);
-  } else {
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.f, (const double*)c.s64.f, c.n_f * c.nrhs);
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.fw, (const double*)c.s64.fw, c.n_fw * c.nrhs);
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_e, (const double*)c.s64.buf_e, c.n_buf * c.nrhs);
-    hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c.stream, c.s32.buf_w, (const double*)c.s64.buf_w, c.n_buf * c.nrhs);
-  }
-  HIP_TRY(hipGetLastError());
+  if (int rc = ctx_dc_flush(c)) return rc;
   if (c.own_stream) HIP_TRY(hipStreamSynchronize(c.stream));
   c.phase = phase;
   return 0;
@@ -1893,6 +2164,16 @@ int piks_set_precision(void* ctx, int32_t precision) {
   return ctx_set_phase(*static_cast<Ctx*>(ctx), precision);
 }
 
+int piks_set_correction_block(void* ctx, int32_t m) {
+  if (!ctx) return fail(PIKS_E_ARG, "bad context");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  if (m < 0 || m > 100000) return fail(PIKS_E_ARG, "block length must be 0..100000");
+  if (!c.r32.f) return fail(PIKS_E_STATE, "context was created without FP32 correction storage");
+  c.dc_fixed = m;
+  if (m > 0) c.dc_len = m;
+  return 0;
+}
+
 static int mass_flow_sync(Ctx& c, double* m) {
   const size_t np = (size_t)c.g.Ny * c.g.Nz;
   std::vector<double> u(np * c.nrhs);
@@ -1933,7 +2214,12 @@ int piks_chk_converge_multi(void* ctx, double* out) {
   Ctx& c = *static_cast<Ctx*>(ctx);
   std::vector<double> m((size_t)c.nrhs);
   if (int rc = mass_flow_sync(c, m.data())) return rc;
-  for (int r = 0; r < c.nrhs; ++r) ctx_check(c, r, m[(size_t)r], out + 3 * r);
+  double worst = 0.0;
+  for (int r = 0; r < c.nrhs; ++r) {
+    ctx_check(c, r, m[(size_t)r], out + 3 * r);
+    worst = std::isfinite(out[3 * r + 2]) ? std::max(worst, out[3 * r + 2]) : HUGE_VAL;
+  }
+  ctx_dc_policy(c, worst);
   return 0;
 }
 
@@ -1942,6 +2228,7 @@ int piks_get_macro_rhs(void* ctx, int32_t rhs, double* rho, double* ux, double* 
   Ctx& c = *static_cast<Ctx*>(ctx);
   if (rhs < 0 || rhs >= c.nrhs) return fail(PIKS_E_ARG, "right-hand side out of range");
   HIP_TRY(hipSetDevice(c.device));
+  if (int rc = ctx_dc_flush(c)) return rc;
   std::vector<Mom> h((size_t)c.g.Ntotal);
   HIP_TRY(hipMemcpyAsync(h.data(), c.mom + (size_t)rhs * c.g.Ntotal, h.size() * sizeof(Mom),
                          hipMemcpyDeviceToHost, c.stream));
@@ -1962,6 +2249,7 @@ int piks_chk_converge(void* ctx, double out[3]) {
   std::vector<double> m((size_t)c.nrhs);
   if (int rc = mass_flow_sync(c, m.data())) return rc;
   ctx_check(c, 0, m[0], out);  // src/output.F90:151-155
+  ctx_dc_policy(c, out[2]);
   return 0;
 }
 
@@ -1969,6 +2257,7 @@ int piks_get_macro(void* ctx, double* rho, double* ux, double* uy, double* uz) {
   if (!ctx) return fail(PIKS_E_ARG, "null context");
   Ctx& c = *static_cast<Ctx*>(ctx);
   HIP_TRY(hipSetDevice(c.device));
+  if (int rc = ctx_dc_flush(c)) return rc;
   std::vector<Mom> h((size_t)c.g.Ntotal);
   HIP_TRY(hipMemcpyAsync(h.data(), c.mom, h.size() * sizeof(Mom), hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1987,15 +2276,9 @@ int piks_get_f(void* ctx, double* f) {
   HIP_TRY(hipSetDevice(c.device));
   const size_t nc8 = c.d.Nc8, nt = (size_t)c.g.Ntotal;
   std::vector<double> h(c.n_f);
-  if (c.phase == PIKS_PREC_FP32_REFINED) {
-    std::vector<float> hf(h.size());
-    HIP_TRY(hipMemcpyAsync(hf.data(), c.s32.f, hf.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
-    HIP_TRY(hipStreamSynchronize(c.stream));
-    for (size_t q = 0; q < h.size(); ++q) h[q] = hf[q];
-  } else {
-    HIP_TRY(hipMemcpyAsync(h.data(), c.s64.f, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIP_TRY(hipStreamSynchronize(c.stream));
-  }
+  if (int rc = ctx_dc_flush(c)) return rc;
+  HIP_TRY(hipMemcpyAsync(h.data(), c.s64.f, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
   // device [o][storage index][l] -> Fortran f(cell, l, o)
   for (size_t o = 0; o < 8; ++o) {
     const std::vector<int32_t>& pos = c.g.pos[ofam((int)o)];
@@ -2014,15 +2297,9 @@ int piks_get_fw(void* ctx, double* fw) {
   const size_t nc8 = c.d.Nc8, ncor = (size_t)c.g.ncorner;
   if (ncor == 0) return 0;
   std::vector<double> h(ncor * 8 * nc8);
-  if (c.phase == PIKS_PREC_FP32_REFINED) {
-    std::vector<float> hf(h.size());
-    HIP_TRY(hipMemcpyAsync(hf.data(), c.s32.fw, hf.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
-    HIP_TRY(hipStreamSynchronize(c.stream));
-    for (size_t q = 0; q < h.size(); ++q) h[q] = hf[q];
-  } else {
-    HIP_TRY(hipMemcpyAsync(h.data(), c.s64.fw, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIP_TRY(hipStreamSynchronize(c.stream));
-  }
+  if (int rc = ctx_dc_flush(c)) return rc;
+  HIP_TRY(hipMemcpyAsync(h.data(), c.s64.fw, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));
   for (size_t o = 0; o < 8; ++o)
     for (size_t q = 0; q < ncor; ++q)
       for (size_t l = 0; l < nc8; ++l) fw[(o * nc8 + l) * ncor + q] = h[(q * 8 + o) * nc8 + l];
@@ -2054,6 +2331,8 @@ int piks_get_info(void* ctx, piks_info* info) {
   info->device_bytes = c.bytes;
   info->coupled_walls = (int32_t)g.coupled.size();
   info->precision = c.phase;
+  info->steps_fp32 = c.dc_inner;
+  info->dc_block = c.phase == PIKS_PREC_FP32_REFINED ? c.dc_len : 0;
   return 0;
 }
 

@@ -51,7 +51,7 @@ struct Request {
   piks_params prm{};
   std::vector<uint8_t> image;
   int32_t max_step = 0;
-  double eps = 0.0, refine_switch = 1e-6;
+  double eps = 0.0;
   int status = PIKS_REQ_QUEUED;
   int error_code = 0;
   std::string error_msg;
@@ -59,10 +59,9 @@ struct Request {
   size_t bytes = 0;
   int slot = -1;
   // solve state (src/main.F90:95-121)
-  int step = 0, steps32 = 0, chunk = 0, stalled = 0;
-  double best = std::numeric_limits<double>::infinity();
+  int step = 0, steps32 = 0, chunk = 0;
   double mass = 1.0, perm = 0.0, error = 1.0;
-  bool converged = false, refined = false;
+  bool converged = false;
   Clock::time_point t_submit, t_admit, t_done;
 };
 
@@ -175,29 +174,17 @@ void fail_request(Server& s, Slot& sl, Request& r, int rc) {
 void on_chunk_done(Server& s, Slot& sl, Request& r) {
   Ctx& c = *r.ctx;
   const int step = r.step + r.chunk;
-  const int steps32 = r.steps32 + (r.refined ? r.chunk : 0);
   const int chk = c.prm.chk_converge_step;
-  bool done = false, converged = false, to_fp64 = false;
+  bool done = false, converged = false;
   double mass = r.mass, perm = r.perm, error = r.error;
   if (step % chk == 0) {
     double out[3];
     ctx_check(c, 0, ctx_mass(c, sl.plane), out);
+    ctx_dc_policy(c, out[2]);  // FP32_REFINED: the next defect-correction blocks
     mass = out[0];
     perm = out[1];
     error = out[2];
-    if (r.refined) {
-      // FP32 has gone as far as it can: below the switch level, or no
-      // improvement for three checks; finish in FP64
-      if (error < r.best) {
-        r.best = error;
-        r.stalled = 0;
-      } else {
-        ++r.stalled;
-      }
-      // the last max(3 checks, max_step / 5) iterations always run in FP64
-      const int tail = std::max(3 * chk, r.max_step / 5);
-      if (error <= std::max(r.refine_switch, r.eps) || r.stalled >= 3 || step >= r.max_step - tail) to_fp64 = true;
-    } else if (error <= r.eps) {
+    if (error <= r.eps) {
       converged = true;
       done = true;
     }
@@ -206,17 +193,11 @@ void on_chunk_done(Server& s, Slot& sl, Request& r) {
   {
     std::lock_guard<std::mutex> lk(s.mu);
     r.step = step;
-    r.steps32 = steps32;
+    r.steps32 = (int)c.dc_inner;
     r.mass = mass;
     r.perm = perm;
     r.error = error;
     r.converged = converged;
-  }
-  if (to_fp64) {
-    if (int rc = ctx_set_phase(c, PIKS_PREC_FP64)) return fail_request(s, sl, r, rc);
-    r.refined = false;
-    std::lock_guard<std::mutex> lk(s.mu);
-    r.error = 1.0;
   }
   if (!done) {
     if (int rc = enqueue_chunk(sl, r)) fail_request(s, sl, r, rc);
@@ -276,7 +257,6 @@ void scheduler(Server* s) {
         r->slot = (int)(free_slot - s->slots.data());
         r->t_admit = Clock::now();
         r->status = PIKS_REQ_RUNNING;
-        r->refined = r->prm.precision == PIKS_PREC_FP32_REFINED;
         s->in_use += r->bytes;
         free_slot->req = r;
         admit.emplace_back(free_slot, r);
@@ -374,7 +354,7 @@ int piks_serve_submit(void* server, const piks_params* p, const uint8_t* raw_ima
   r->image.assign(raw_image, raw_image + image_bytes);
   r->max_step = max_step;
   r->eps = eps;
-  r->refine_switch = refine_switch > 0 ? refine_switch : 1e-6;
+  (void)refine_switch;  // FP32_REFINED requests run the defect correction
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> lk(s.mu);
   r->ticket = s.next_ticket++;

@@ -16,10 +16,10 @@
 !             exact_order = F, refineSwitch = 1.0d-6 /
 ! exact_order = T sums the moments and wall fluxes in the reference's order,
 ! so every value (and every output file) equals the reference's bit for bit.
-! With precision = 'fp32' the iterations run in FP32 until the check's error
-! is below refineSwitch, has stopped falling for three checks, or the last
-! max(3 checks, maxStep / 5) iterations begin; the rest run in FP64, so the
-! converged permeability is the FP64 solve's.
+! With precision = 'fp32' the library runs its FP64 defect correction (FP32
+! correction iterations between FP64 residual iterations, tracking the FP64
+! iterates; see piks_set_precision in include/piks3d.h); refineSwitch is
+! read and ignored.
 program piks3d_gpu
     use iso_c_binding
     use piks_c
@@ -55,9 +55,7 @@ program piks3d_gpu
     double precision, allocatable, target :: rho(:), ux(:), uy(:), uz(:)
     integer(c_int32_t), allocatable, target :: labels(:)
     integer :: ios, rc, iStep, nstep, nbytes, next_stop
-    double precision :: error, mass, permeability, out(3), best
-    logical :: refined
-    integer :: stalled
+    double precision :: error, mass, permeability, out(3)
     integer(8) :: t0, t1, trate
 
     ! ------------------------------------------------------------ para.in
@@ -145,9 +143,6 @@ program piks3d_gpu
     error = 1.d0
     permeability = 0.d0
     mass = 1.d0
-    refined = info%precision == PIKS_PREC_FP32_REFINED
-    best = huge(1.d0)
-    stalled = 0
     call system_clock(t0, trate)
     iStep = 0
     do while (iStep < maxStep)
@@ -160,24 +155,6 @@ program piks3d_gpu
         iStep = next_stop
         if (mod(iStep, chkConvergeStep) == 0) call chk_converge()
         if (mod(iStep, saveStep) == 0) call save_fields()
-        if (refined) then
-            if (mod(iStep, chkConvergeStep) /= 0) cycle
-            if (error < best) then
-                best = error
-                stalled = 0
-            else
-                stalled = stalled + 1
-            endif
-            ! the last max(3 checks, maxStep / 5) iterations always run in FP64
-            if (error <= max(refineSwitch, eps) .or. stalled >= 3 .or. &
-                iStep >= maxStep - max(3 * chkConvergeStep, maxStep / 5)) then
-                rc = piks_set_precision(ctx, PIKS_PREC_FP64)
-                call check_rc(rc)
-                refined = .false.
-                error = 1.d0
-            endif
-            cycle
-        endif
         if (error <= eps) then
             if (iStep < chkConvergeStep) call chk_converge()
             exit

@@ -40,6 +40,9 @@ module piks_c
         integer(c_int64_t) :: device_bytes
         integer(c_int32_t) :: coupled_walls
         integer(c_int32_t) :: precision
+        integer(c_int64_t) :: steps_fp32
+        integer(c_int32_t) :: dc_block
+        integer(c_int32_t) :: reserved
     end type piks_info
 
     interface
@@ -62,6 +65,11 @@ module piks_c
             type(c_ptr), value :: ctx
             integer(c_int32_t), value :: precision
         end function piks_set_precision
+        integer(c_int) function piks_set_correction_block(ctx, m) bind(C, name='piks_set_correction_block')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: m
+        end function piks_set_correction_block
 
         integer(c_int) function piks_chk_converge(ctx, out) bind(C, name='piks_chk_converge')
             import :: c_int, c_double, c_ptr

@@ -175,6 +175,7 @@ def run_case(name, factory, over, dump_args, subset, threads=1):
                   vecwall=rd("setup_vecWall.bin", np.int32),
                   extcoef=rd("setup_extCoef.bin"),
                   mapf=rd("setup_mapF.bin", np.int32))
+    if subset != "velocity":
         conv = np.fromfile(os.path.join(out, "conv.bin"),
                            dtype=np.dtype([("step", "<i4"), ("mass", "<f8"), ("perm", "<f8"),
                                            ("error", "<f8")]))

@@ -122,14 +122,17 @@ def test_ragged_sides_wall_order_3():
     _compare(p, _boxes(13, 7, 5, seed=11), True, p.max_step)
 
 
-def test_refined_precision_on_a_multi_pass_grid():
-    """FP32 storage on a 216-velocity-per-octant grid: after the same number
-    of iterations the moments stay within single-precision distance of the
-    FP64 path, and switching to FP64 mid-way leaves the iteration running."""
+def test_refined_precision_on_a_multi_pass_grid(monkeypatch):
+    """Defect correction on a 216-velocity-per-octant grid (blocks of 3: one
+    FP64 iteration, two FP32 corrections): after the same number of
+    iterations the moments stay within single-precision distance of the FP64
+    path, and switching to FP64 mid-block folds the correction and leaves the
+    iteration running."""
     print("case refined nc=6", flush=True)
     from piks3d_amd import PIKS_PREC_FP64, Solver
     p = _params(9, 10, 11, 6, steps=6)
     img = _boxes(9, 10, 11, seed=6)
+    monkeypatch.setenv("PIKS_DC_BLOCK", "3")
     with Solver(p, img, precision="fp64") as a, Solver(p, img, precision="fp32") as b:
         a.iterate(4)
         b.iterate(4)
