@@ -64,7 +64,12 @@ typedef struct piks_params {
     int32_t exact_order;           /* 1: wall-flux and moment sums run serially in the
                                       reference's order, so every iteration is bit-for-bit
                                       the reference's (slower; for verification) */
-    int32_t reserved[4];
+    int32_t sweep;                 /* sweep schedule: 0 the library's default, 1 one launch per
+                                      hyperplane, 2 tiles (one launch per tile diagonal; grids
+                                      of at most 64 velocities per octant, FP64 contexts) */
+    int32_t tile;                  /* sweep = 2: tx | ty << 8 | tz << 16 (tx * ty one of
+                                      4x4, 4x2, 8x2, 2x2; 0: 2x2x2) */
+    int32_t reserved[2];
 } piks_params;
 
 typedef struct piks_info {

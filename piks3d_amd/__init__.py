@@ -40,7 +40,7 @@ class PiksParams(ctypes.Structure):
         ("kn", ctypes.c_double), ("press_drop", ctypes.c_double), ("accom", ctypes.c_double),
         ("precision", ctypes.c_int32), ("device", ctypes.c_int32),
         ("exact_quadrature", ctypes.c_int32), ("exact_order", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("sweep", ctypes.c_int32), ("tile", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -161,6 +161,8 @@ def to_cparams(p: Params, precision=None, exact_order=None, device=None) -> Piks
     cp.device = p.device if device is None else device
     cp.exact_quadrature = int(bool(p.exact_quadrature))
     cp.exact_order = int(bool(p.exact_order if exact_order is None else exact_order))
+    cp.sweep = int(p.sweep)
+    cp.tile = int(p.tile)
     return cp
 
 
@@ -310,7 +312,12 @@ class Server:
     concurrently.  Each result is the same as a Solver running the same
     request alone."""
 
-    def __init__(self, device=0, max_slots=8, prep_threads=4, use_graphs=True, memory_budget=0):
+    def __init__(self, device=0, max_slots=8, prep_threads=4, use_graphs=True, memory_budget=0, tuning="auto"):
+        """tuning: the sweep-schedule table of piks3d_amd.autotune applied to
+        requests that choose no schedule ("auto": the package's tuning.json
+        when present, None: none, or a path)."""
+        from . import autotune
+        self.tuning = autotune.load(autotune.TABLE if tuning == "auto" else tuning) if tuning else None
         sp = ServeParams()
         sp.device, sp.max_slots, sp.prep_threads = device, max_slots, prep_threads
         sp.use_graphs = int(bool(use_graphs))
@@ -324,6 +331,10 @@ class Server:
         returns its ticket."""
         raw = np.ascontiguousarray(np.asarray(image, dtype=np.uint8).ravel())
         cp = to_cparams(params, precision)
+        if cp.sweep == 0 and self.tuning:
+            from . import autotune
+            cp.sweep, cp.tile = autotune.choose(self.tuning, max(params.nx, params.ny, params.nz),
+                                                params.nc_fundamental)
         t = ctypes.c_int64()
         _check(lib().piks_serve_submit(
             self._h, ctypes.byref(cp), raw.ctypes.data, raw.size,

@@ -112,6 +112,8 @@ struct Ctx {
                                // workgroup ends with its wave, measured fastest)
   bool sweep_recip = false;    // fast mode: multiply by a tabulated 1 / denominator
   bool sweep_generic = false;  // single solve through the multi-right-hand-side kernel
+  bool planes_fused = true;    // inlet/outlet and symmetry planes in one launch (k_planes)
+  int32_t* surface = nullptr;  // Geometry::surface
   double* inv_den = nullptr;   // [first-order flags][velocity]
   SweepCoef* coef = nullptr;   // [first-order flags]
   int32_t* hp_task = nullptr;

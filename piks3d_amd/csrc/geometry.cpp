@@ -316,6 +316,11 @@ int build_geometry(const GeomInput& in, const uint8_t* raw, int64_t nbytes, Geom
     g.fluid_boundary.insert(g.fluid_boundary.end(), plane_bnd[(size_t)k].begin(), plane_bnd[(size_t)k].end());
   }
   g.nfluid = (int64_t)(g.fluid_interior.size() + g.fluid_boundary.size());
+  for (int k = 1; k <= g.Nz; ++k)
+    for (int j = 1; j <= g.Ny; ++j)
+      for (int i = 1; i <= g.Nx; ++i)
+        if (i == 1 || i == g.Nx || j == 1 || j == g.Ny || k == 1 || k == g.Nz)
+          g.surface.push_back((int32_t)g.cell(i, j, k));
 
   // walls whose extrapolation reads another wall (see geometry.h)
   std::vector<int> level(nwall, 0);

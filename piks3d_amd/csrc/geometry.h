@@ -75,6 +75,7 @@ struct Geometry {
   std::vector<int32_t> fluid_interior;    // not on any boundary plane: moments from sweep partials
   std::vector<int32_t> fluid_boundary;    // on i=1,Nx / j=1,Ny / k=1,Nz planes
   std::vector<int32_t> commit_walls;      // walls with a switch in some octant
+  std::vector<int32_t> surface;           // cells on the surface of [1,Nx] x [1,Ny] x [1,Nz]
   int64_t nfluid = 0;
 
   // Storage of f.  Per octant only the cells that some kernel writes are

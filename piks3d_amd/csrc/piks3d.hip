@@ -1201,6 +1201,53 @@ __global__ void k_sym_z(T* f, Dims d) {
   }
 }
 
+// The three plane kernels above in one launch: one wave per cell on the
+// surface of the box [1, Nx] x [1, Ny] x [1, Nz] (the union of their ranges),
+// one velocity per lane, applying in the reference's statement order the
+// inlet or outlet (src/solver.F90:1832-1862), the south or north mirror
+// (:1867-1892) and the back or front mirror (:1893-1918) that apply to the
+// cell.  Each step reads and writes the cell itself (the inlet/outlet also
+// reads the x ghost written by the exchange before), so the cells are
+// independent and the result is that of the three kernels in sequence.
+template <typename T>
+__global__ __launch_bounds__(256) void k_planes(T* f, const double* w, double pd, const int32_t* cells, int64_t n,
+                                                Dims d) {
+  f += blockIdx.y * d.rs_f;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wpb = blockDim.x / kWave;
+  for (int64_t it = (int64_t)blockIdx.x * wpb + wave_in_block(); it < n; it += (int64_t)gridDim.x * wpb) {
+    const int64_t c = cells[it];
+    const int k = (int)(c / d.Nxyt) - 1;
+    const int j = (int)((c % d.Nxyt) / d.Nxt) - 1;
+    const int i = (int)(c % d.Nxt) - 1;
+    for (int l = lane; l < d.Nc8; l += kWave) {
+      const double wpd = w[l] * pd;
+      if (i == 1) {
+        // inlet: octants with cx > 0 from the ghost x = 0
+        const T a0 = f[fat(d, 0, c - 1) + l], a3 = f[fat(d, 3, c - 1) + l];
+        const T a4 = f[fat(d, 4, c - 1) + l], a7 = f[fat(d, 7, c - 1) + l];
+        f[fat(d, 0, c) + l] = (T)((double)a0 + wpd);
+        f[fat(d, 3, c) + l] = (T)((double)a3 + wpd);
+        f[fat(d, 4, c) + l] = (T)((double)a4 + wpd);
+        f[fat(d, 7, c) + l] = (T)((double)a7 + wpd);
+      }
+      if (i == d.Nx) {
+        // outlet: octants with cx < 0 from the ghost x = Nx + 1
+        const T b1 = f[fat(d, 1, c + 1) + l], b2 = f[fat(d, 2, c + 1) + l];
+        const T b5 = f[fat(d, 5, c + 1) + l], b6 = f[fat(d, 6, c + 1) + l];
+        f[fat(d, 1, c) + l] = (T)((double)b1 - wpd);
+        f[fat(d, 2, c) + l] = (T)((double)b2 - wpd);
+        f[fat(d, 5, c) + l] = (T)((double)b5 - wpd);
+        f[fat(d, 6, c) + l] = (T)((double)b6 - wpd);
+      }
+      if (j == 1) mirror4<T>(f, d, c, l, 1, 2, 0, 3, 5, 6, 4, 7);
+      if (j == d.Ny) mirror4<T>(f, d, c, l, 2, 1, 3, 0, 6, 5, 7, 4);
+      if (k == 1) mirror4<T>(f, d, c, l, 0, 4, 1, 5, 2, 6, 3, 7);
+      if (k == d.Nz) mirror4<T>(f, d, c, l, 4, 0, 5, 1, 6, 2, 7, 3);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ moments
 
 // Interior fluid cells: their f is final after the sweep, so the moments are
@@ -1605,12 +1652,18 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
       hipLaunchKernelGGL(k_walls_coupled<T>, dim3(grid_for(n, wpb), R), dim3(wpb * kWave), 0,
                          c.stream, wa);
     }
-    hipLaunchKernelGGL(k_inout<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Ny * g.Nz * 4, wpb) : grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256), R), dim3(256),
-                       0, c.stream, st.f, (const double*)c.dw, im.pd, d);
-    hipLaunchKernelGGL(k_sym_y<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Nz, wpb) : grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256), R), dim3(256), 0,
-                       c.stream, st.f, d);
-    hipLaunchKernelGGL(k_sym_z<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Ny, wpb) : grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256), R), dim3(256), 0,
-                       c.stream, st.f, d);
+    if (c.planes_fused) {
+      hipLaunchKernelGGL(k_planes<T>, dim3(grid_for((int64_t)g.surface.size(), wpb), R), dim3(wpb * kWave), 0,
+                         c.stream, st.f, (const double*)c.dw, im.pd, (const int32_t*)c.surface,
+                         (int64_t)g.surface.size(), d);
+    } else {
+      hipLaunchKernelGGL(k_inout<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Ny * g.Nz * 4, wpb) : grid_for((int64_t)g.Ny * g.Nz * 4 * d.Nc8, 256), R), dim3(256),
+                         0, c.stream, st.f, (const double*)c.dw, im.pd, d);
+      hipLaunchKernelGGL(k_sym_y<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Nz, wpb) : grid_for((int64_t)g.Nx * g.Nz * d.Nc8, 256), R), dim3(256), 0,
+                         c.stream, st.f, d);
+      hipLaunchKernelGGL(k_sym_z<T>, dim3(d.Nc8 <= kWave ? grid_for((int64_t)g.Nx * g.Ny, wpb) : grid_for((int64_t)g.Nx * g.Ny * d.Nc8, 256), R), dim3(256), 0,
+                         c.stream, st.f, d);
+    }
     if (tm) HIP_TRY(hipEventRecord(c.ev[ev + 2], c.stream));
     if (c.exact) {
       const int64_t n = (int64_t)g.fluid_interior.size() + (int64_t)g.fluid_boundary.size();
@@ -1771,6 +1824,22 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
     c->tile_tx = tx; c->tile_ty = ty; c->tile_tz = tz;
   }
   if (const char* b = std::getenv("PIKS_GRAPH")) c->use_graph = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_PLANES_FUSED")) c->planes_fused = std::atoi(b) != 0;
+  // the sweep schedule of the parameters (a tuning table's choice) takes
+  // precedence over the environment
+  if (p->sweep == 1) {
+    c->sweep_tiles = false;
+  } else if (p->sweep == 2) {
+    c->sweep_tiles = true;
+    if (p->tile != 0) {
+      const int tx = p->tile & 255, ty = (p->tile >> 8) & 255, tz = (p->tile >> 16) & 0xffff;
+      if (!tile_kernel<double>(tx, ty) || tz < 2 || tz > 256)
+        return fail(PIKS_E_ARG, "tile must be tx | ty << 8 | tz << 16 with tx x ty one of 4x4, 4x2, 8x2, 2x2");
+      c->tile_tx = tx; c->tile_ty = ty; c->tile_tz = tz;
+    }
+  } else if (p->sweep != 0) {
+    return fail(PIKS_E_ARG, "sweep must be 0, 1 or 2");
+  }
   if (const char* b = std::getenv("PIKS_DC_BLOCK")) {
     const int v = std::atoi(b);
     if (v < 1 || v > 100000) return fail(PIKS_E_ARG, "PIKS_DC_BLOCK must be 1..100000");
@@ -1845,7 +1914,7 @@ size_t ctx_device_bytes(const Ctx& c) {
   const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.wall_pos.size() * 4 +
                        g.uncoupled.size() * 4 +
                        g.coupled.size() * 4 * 7 + g.snap_cell.size() * 4 + g.commit_walls.size() * 4;
-  const size_t fluid = (size_t)(g.fluid_interior.size() + g.fluid_boundary.size()) * 8;
+  const size_t fluid = (size_t)(g.fluid_interior.size() + g.fluid_boundary.size()) * 8 + g.surface.size() * 4;
   const size_t sched = c.sweep_tiles ? g.tl_rec.size() * 4 : g.hp_task.size() * 4 + g.hp_lo.size() * 16;
   return state64 + state32 + per_cell + walls + fluid + sched +
          R * ((size_t)g.Ny * g.Nz * 8 + 8) + 4 * c.cx.size() * 8 + 64 * 256;
@@ -1926,7 +1995,8 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
       (rc = upload(c, &c.e2, g.wall_e2)) || (rc = upload(c, &c.uncoupled, g.uncoupled)) || (rc = upload(c, &c.wall_pos, g.wall_pos)) ||
       (rc = upload(c, &c.coupled, g.coupled)) || (rc = upload(c, &c.coupled_snap, g.coupled_snap)) ||
       (rc = upload(c, &c.snap_cell, g.snap_cell)) || (rc = upload(c, &c.commit_walls, g.commit_walls)) ||
-      (rc = upload(c, &c.fluid_int, g.fluid_interior)) || (rc = upload(c, &c.fluid_bnd, g.fluid_boundary)))
+      (rc = upload(c, &c.fluid_int, g.fluid_interior)) || (rc = upload(c, &c.fluid_bnd, g.fluid_boundary)) ||
+      (rc = upload(c, &c.surface, g.surface)))
     return rc;
   std::vector<int32_t> all(g.fluid_interior);
   all.insert(all.end(), g.fluid_boundary.begin(), g.fluid_boundary.end());

@@ -125,6 +125,8 @@ program piks3d_gpu
     prm%device = device
     prm%exact_quadrature = merge(1, 0, exact_quadrature)
     prm%exact_order = merge(1, 0, exact_order)
+    prm%sweep = 0
+    prm%tile = 0
     prm%reserved = 0
     rc = piks_create(prm, c_loc(raw), int(nbytes, c_int64_t), ctx)
     if (rc /= 0) then

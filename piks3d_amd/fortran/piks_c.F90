@@ -24,7 +24,8 @@ module piks_c
         integer(c_int32_t) :: device
         integer(c_int32_t) :: exact_quadrature
         integer(c_int32_t) :: exact_order
-        integer(c_int32_t) :: reserved(4)
+        integer(c_int32_t) :: sweep, tile
+        integer(c_int32_t) :: reserved(2)
     end type piks_params
 
     type, bind(C) :: piks_info

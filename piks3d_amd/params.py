@@ -53,6 +53,10 @@ class Params:
     exact_quadrature: bool = False
     exact_order: bool = False
     refine_switch: float = 1.0e-6
+    # sweep schedule (piks_params.sweep / tile): 0 the library's default,
+    # 1 hyperplanes, 2 tiles of tile = tx | ty << 8 | tz << 16
+    sweep: int = 0
+    tile: int = 0
 
     # sizes after the fluid layers are added (src/parameters.F90:121)
     @property

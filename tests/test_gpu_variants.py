@@ -1,13 +1,15 @@
 """The sweep's launch variants give the default path's results.
 
-A context reads eight environment settings at creation (piks3d.hip,
+A context reads nine environment settings at creation (piks3d.hip,
 ctx_create_host; geometry.cpp): PIKS_HOST_THREADS (host threads of the image
 preprocessing, whose loops write disjoint data), PIKS_XCD_REMAP
 (XCD-contiguous task ranges), PIKS_SWEEP_BLOCK (64 or 256 threads per sweep
 workgroup), PIKS_GRAPH (one HIP graph per
 iteration), PIKS_SWEEP_GENERIC (the multi-right-hand-side kernel for a single
 solve), PIKS_SWEEP_TILES and PIKS_TILE (the tiled sweep, one wave per tile of
-one octant walking its planes, and its tile shape) and PIKS_SWEEP_RECIP (fast
+one octant walking its planes, and its tile shape), PIKS_PLANES_FUSED (0: the
+inlet/outlet and the two symmetry directions as three launches instead of
+one) and PIKS_SWEEP_RECIP (fast
 mode: multiply by a tabulated reciprocal of the denominator).  All but the
 last compute every update with the same
 operations in the same order, so f, fw and the moments after several
@@ -33,6 +35,7 @@ VARIANTS = [
     {"PIKS_SWEEP_TILES": "1"},
     {"PIKS_SWEEP_TILES": "1", "PIKS_TILE": "4x4x16"},
     {"PIKS_SWEEP_TILES": "1", "PIKS_TILE": "8x2x3", "PIKS_GRAPH": "1"},
+    {"PIKS_PLANES_FUSED": "0"},
 ]
 CASES = ["boxes_n2_a1_e2", "spheres40_n2"]
 STEPS = 4
@@ -41,7 +44,7 @@ STEPS = 4
 def _run(p, img, exact, env, monkeypatch):
     from piks3d_amd import Solver
     for k in ("PIKS_XCD_REMAP", "PIKS_SWEEP_BLOCK", "PIKS_GRAPH", "PIKS_SWEEP_GENERIC", "PIKS_SWEEP_RECIP",
-              "PIKS_HOST_THREADS", "PIKS_SWEEP_TILES", "PIKS_TILE"):
+              "PIKS_HOST_THREADS", "PIKS_SWEEP_TILES", "PIKS_TILE", "PIKS_PLANES_FUSED"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
