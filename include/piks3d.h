@@ -137,6 +137,17 @@ int piks_chk_converge(void* ctx, double out[3]);
  * (massLocal of src/output.F90:140-142). */
 int piks_mass_flow(void* ctx, double* mass);
 
+/* piks_mass_flow for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops): no counterpart in the reference.  The same
+ * value, bit for bit, written to the device double mass_dev; enqueued on the
+ * context's stream without synchronising and without allocating anything
+ * (no hipMalloc, no host allocation): the plane is staged in a fixed device
+ * array of the library of 4 MiB per device (at most 524288 cells, Ny * Nz),
+ * larger planes fail with PIKS_E_ARG.  The array is shared by every call on
+ * a device and keeps the last call's plane: calls that overlap on one device
+ * (several threads or streams) race. */
+int piks_mass_flow_noalloc(void* ctx, double* mass_dev);
+
 /* Macroscopic fields Rho, Ux, Uy, Uz (src/flow.F90:72), Ntotal doubles each,
  * as the reference's writers read them (src/output.F90:176-316).  NULL
  * pointers are skipped. */

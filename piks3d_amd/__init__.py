@@ -89,6 +89,7 @@ _API = {
     "piks_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "piks_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "piks_set_correction_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "piks_mass_flow_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "piks_chk_converge": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_mass_flow": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "piks_get_macro": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4),

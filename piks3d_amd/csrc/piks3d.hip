@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1337,6 +1338,34 @@ __global__ void k_gather_plane(const Mom* mom, const Mom* mom_e, double* out, Di
   }
 }
 
+// ------------------------------------------------------------------ allocation-free mass flow
+
+// Device scratch of piks_mass_flow_noalloc: the Ux plane x = 2 of right-hand
+// side 0 (Ny * Nz doubles, at most kScratchDoubles).  One copy per device,
+// held until the process ends; host code uses the address hipGetSymbolAddress
+// returns for the current device, never this name.
+constexpr size_t kScratchDoubles = (size_t)1 << 19;  // 4 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// massLocal of src/output.F90:80-142 on one thread, in the reference's
+// summation order (the same operations as ctx_mass on the host, so the same
+// bits): u is the plane, j fastest; sq = sqrt(1/2), rl = 1 / Ref_L.
+__global__ void k_mass_serial(const double* u, int Ny, int Nz, double ds, double sq, double pd, double rl,
+                              double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  auto U = [&](int j, int k) { return u[(size_t)(j - 1) + (size_t)(k - 1) * Ny]; };
+  double inner = 0.0, suth = 0.0, noth = 0.0, frnt = 0.0, back = 0.0;
+  for (int k = 2; k <= Nz - 1; ++k)
+    for (int j = 2; j <= Ny - 1; ++j) inner = inner + U(j, k) * ds * ds;
+  for (int k = 2; k <= Nz - 1; ++k) suth = suth + U(1, k) * ds * ds * 0.5;
+  for (int k = 2; k <= Nz - 1; ++k) noth = noth + U(Ny, k) * ds * ds * 0.5;
+  for (int j = 2; j <= Ny - 1; ++j) back = back + U(j, 1) * ds * ds * 0.5;
+  for (int j = 2; j <= Ny - 1; ++j) frnt = frnt + U(j, Nz) * ds * ds * 0.5;
+  const double sb = U(1, 1) * ds * ds * 0.25, sf = U(1, Nz) * ds * ds * 0.25;
+  const double nb = U(Ny, 1) * ds * ds * 0.25, nf = U(Ny, Nz) * ds * ds * 0.25;
+  *out = (inner + suth + noth + frnt + back + sb + sf + nb + nf) * sq / pd / (rl * rl);
+}
+
 // ------------------------------------------------------------------ context
 
 // ------------------------------------------------------------------ defect correction
@@ -2250,6 +2279,42 @@ static int mass_flow_sync(Ctx& c, double* m) {
   if (int rc = ctx_enqueue_plane(c, u.data())) return rc;
   HIP_TRY(hipStreamSynchronize(c.stream));
   for (int r = 0; r < c.nrhs; ++r) m[r] = ctx_mass(c, u.data() + np * r);
+  return 0;
+}
+
+// Per-device address of `scratch`, taken once per device with that device
+// current (the array's host-side name is not a device address).
+static double* scratch_on(int device) {
+  static std::mutex mu;
+  static double* ptr[256] = {};
+  static bool done[256] = {};
+  if (device < 0 || device >= 256) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!done[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    ptr[device] = static_cast<double*>(p);
+    done[device] = true;
+  }
+  return ptr[device];
+}
+
+int piks_mass_flow_noalloc(void* ctx, double* mass_dev) {
+  if (!ctx || !mass_dev) return fail(PIKS_E_ARG, "null argument");
+  Ctx& c = *static_cast<Ctx*>(ctx);
+  const Geometry& g = c.g;
+  const size_t np = (size_t)g.Ny * g.Nz;
+  if (np > kScratchDoubles)
+    return fail(PIKS_E_ARG, "plane of " + std::to_string(np) + " cells exceeds the allocation-free limit of " +
+                                std::to_string(kScratchDoubles) + " (use piks_mass_flow)");
+  HIP_TRY(hipSetDevice(c.device));
+  double* plane = scratch_on(c.device);
+  if (!plane) return fail(PIKS_E_HIP, "hipGetSymbolAddress of the mass-flow scratch failed");
+  hipLaunchKernelGGL(k_gather_plane, dim3(grid_for((int64_t)np, 256), 1u), dim3(256), 0, c.stream,
+                     (const Mom*)c.mom, c.dc_pending ? (const Mom*)c.mom_e : nullptr, plane, c.d, 2);
+  hipLaunchKernelGGL(k_mass_serial, dim3(1), dim3(64), 0, c.stream, (const double*)plane, g.Ny, g.Nz, g.ds,
+                     std::sqrt(1.0 / 2.0), c.prm.press_drop, 1.0 / c.prm.ref_l, mass_dev);
+  HIP_TRY(hipGetLastError());
   return 0;
 }
 
