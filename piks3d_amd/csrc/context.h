@@ -150,10 +150,12 @@ struct Ctx {
   int dc_len = 1, dc_pos = 0, dc_fixed = 0;
   bool dc_pending = false;
   int64_t dc_inner = 0;              // FP32 correction iterations run
-  // one iteration captured as a HIP graph, per precision (used by ctx_enqueue
-  // when use_graph is set)
+  // one iteration of each kind (plain FP64, the defect-correction block's
+  // FP64 iteration without and with folding, the first and later correction
+  // iterations) captured as a HIP graph (used by ctx_enqueue when use_graph
+  // is set)
   bool use_graph = false;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // per iteration kind
   // timing: four events per step of the last timed call
   std::vector<hipEvent_t> ev;
   double last_ms[4] = {0, 0, 0, 0};

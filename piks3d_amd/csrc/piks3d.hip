@@ -1750,33 +1750,82 @@ int dc_arrays(Ctx& c, int mode, int flag) {
   return 0;
 }
 
+// One iteration of a kind: kIterPlain, or, in a refined context's
+// defect-correction block (see Ctx), the block's FP64 iteration (without or
+// with folding the previous block's correction) or a correction iteration
+// (the block's first or a later one), timed at event step ev (ev < 0:
+// untimed).
+enum { kIterPlain = 0, kIterBlock = 1, kIterBlockFold = 2, kIterCorrFirst = 3, kIterCorr = 4, kIterKinds = 5 };
+
+int iteration_body(Ctx& c, int kind, int ev) {
+  switch (kind) {
+    case kIterPlain:
+      return enqueue_t<double>(c, c.s64, 1, ev, IterMode{c.mom, c.prm.press_drop, 0, 0, 0});
+    case kIterBlock:
+    case kIterBlockFold: {
+      // keep the state, iterate in FP64, take the residual
+      const int fold = kind == kIterBlockFold ? 1 : 0;
+      if (int rc = dc_arrays(c, 0, fold)) return rc;
+      if (int rc = enqueue_t<double>(c, c.s64, 1, ev, IterMode{c.mom, c.prm.press_drop, 1, fold, 0})) return rc;
+      return dc_arrays(c, 1, 0);
+    }
+    default: {
+      // e <- A(e + r), pressure drop 0
+      const int first = kind == kIterCorrFirst ? 1 : 0;
+      if (int rc = dc_arrays(c, 2, first)) return rc;
+      return enqueue_t<float>(c, c.s32, 1, ev, IterMode{c.mom_e, 0.0, 2, 0, first});
+    }
+  }
+}
+
+// Capture one iteration of a kind as a graph (kernel launches only: no
+// memset or allocation node).
+int capture_graph(Ctx& c, int kind) {
+  if (c.graph[kind]) return 0;
+  hipGraph_t gr = nullptr;
+  HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  const int rc = iteration_body(c, kind, -1);
+  const hipError_t e = hipStreamEndCapture(c.stream, &gr);
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PIKS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&c.graph[kind], gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (ei != hipSuccess) return fail(PIKS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return 0;
+}
+
+// One iteration of a kind: replayed from its graph when the context uses
+// graphs and the iteration is untimed, else launched.
+int run_iteration(Ctx& c, int kind, int ev) {
+  if (c.use_graph && ev < 0) {
+    if (int rc = capture_graph(c, kind)) return rc;
+    HIP_TRY(hipGraphLaunch(c.graph[kind], c.stream));
+    return 0;
+  }
+  return iteration_body(c, kind, ev);
+}
+
 // nsteps iterations of a refined context (defect correction, see Ctx),
 // timed from event step ev0 on (ev0 < 0: untimed).
 int dc_iterate(Ctx& c, int nsteps, int ev0) {
-  const IterMode plain{c.mom, c.prm.press_drop, 0, 0, 0};
   for (int step = 0; step < nsteps; ++step) {
     const int ev = ev0 >= 0 ? ev0 + step : -1;
     if (c.dc_len <= 1) {
       // plain FP64 iterations
       if (int rc = ctx_dc_flush(c)) return rc;
-      if (int rc = enqueue_t<double>(c, c.s64, 1, ev, plain)) return rc;
+      if (int rc = run_iteration(c, kIterPlain, ev)) return rc;
       continue;
     }
     if (c.dc_pos == 0 || c.dc_pos >= c.dc_len) {
-      // a block's FP64 iteration: fold the previous block's correction, keep
-      // the state, iterate, take the residual
-      const int fold = c.dc_pending ? 1 : 0;
-      if (int rc = dc_arrays(c, 0, fold)) return rc;
-      if (int rc = enqueue_t<double>(c, c.s64, 1, ev, IterMode{c.mom, c.prm.press_drop, 1, fold, 0})) return rc;
-      if (int rc = dc_arrays(c, 1, 0)) return rc;
+      if (int rc = run_iteration(c, c.dc_pending ? kIterBlockFold : kIterBlock, ev)) return rc;
       c.dc_pending = false;
       c.dc_pos = 1;
       continue;
     }
-    // a correction iteration: e <- A(e + r), pressure drop 0
-    const int first = c.dc_pos == 1 ? 1 : 0;
-    if (int rc = dc_arrays(c, 2, first)) return rc;
-    if (int rc = enqueue_t<float>(c, c.s32, 1, ev, IterMode{c.mom_e, 0.0, 2, 0, first})) return rc;
+    if (int rc = run_iteration(c, c.dc_pos == 1 ? kIterCorrFirst : kIterCorr, ev)) return rc;
     c.dc_pending = true;
     ++c.dc_inner;
     if (++c.dc_pos >= c.dc_len) c.dc_pos = 0;
@@ -2066,32 +2115,12 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
   return 0;
 }
 
-// Capture one plain FP64 iteration as a graph.  The captured work has no
-// memset or allocation node: only kernel launches.
-static int capture_graph(Ctx& c) {
-  const int k = 0;
-  if (c.graph[k]) return 0;
-  hipGraph_t gr = nullptr;
-  HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_t<double>(c, c.s64, 1, -1, IterMode{c.mom, c.prm.press_drop, 0, 0, 0});
-  const hipError_t e = hipStreamEndCapture(c.stream, &gr);
-  if (rc) {
-    if (gr) (void)hipGraphDestroy(gr);
-    return rc;
-  }
-  if (e != hipSuccess) return fail(PIKS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  const hipError_t ei = hipGraphInstantiate(&c.graph[k], gr, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(gr);
-  if (ei != hipSuccess) return fail(PIKS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-  return 0;
-}
-
 int ctx_enqueue(Ctx& c, int nsteps) {
   HIP_TRY(hipSetDevice(c.device));
   if (c.phase == PIKS_PREC_FP32_REFINED) return dc_iterate(c, nsteps, -1);
   if (c.use_graph) {
-    if (int rc = capture_graph(c)) return rc;
-    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c.graph[0], c.stream));
+    for (int s = 0; s < nsteps; ++s)
+      if (int rc = run_iteration(c, kIterPlain, -1)) return rc;
     return 0;
   }
   return enqueue_t<double>(c, c.s64, nsteps, -1, IterMode{c.mom, c.prm.press_drop, 0, 0, 0});

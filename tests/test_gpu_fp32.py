@@ -106,3 +106,24 @@ def test_precision_switch_rules(monkeypatch):
         assert np.abs(fa - fb).max() <= 1e-6 * np.abs(fa).max()
         b.set_precision("fp32")
         assert np.array_equal(b.f(), fb)         # switching back changes nothing stored
+
+
+def test_correction_iterations_replayed_from_graphs(monkeypatch):
+    # every iteration kind of a block (FP64 with and without folding, first
+    # and later correction) replayed from its captured graph gives the
+    # launched iterations' bits; one iteration per call with the check (its
+    # own kernel and a synchronisation) between replays
+    p, img = _pack(n=24)
+    monkeypatch.setenv("PIKS_DC_BLOCK", "4")
+    runs = []
+    for graph in ("0", "1"):
+        monkeypatch.setenv("PIKS_GRAPH", graph)
+        with Solver(p, img, precision="fp32") as s:
+            lines = []
+            for _ in range(11):
+                s.iterate(1)
+                lines.append(s.mass_flow())
+            s.set_precision("fp64")
+            runs.append((lines, s.f()))
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
