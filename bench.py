@@ -115,7 +115,11 @@ def refined_measurement(p, img, steps, warmup, block, k64, fp64_ms_per_step):
     whole-iteration rate of the timed steps, the correction sweep's roofline
     at its algorithmic 12 B per update (read the correction and the residual,
     write the correction) and K against the plain FP64 run at equal
-    iteration counts."""
+    iteration counts.  The blocks are forced from the start of the solve,
+    where a block's correction is as large as the state itself, so this K
+    difference shows FP32 rounding of the whole state (about 1e-7); the
+    library's own block policy stays in FP64 until the check's error is below
+    1e-2 and is held to 1e-8 in K by the tests."""
     from piks3d_amd import Solver
     os.environ["PIKS_DC_BLOCK"] = "1"
     with Solver(p, img, precision="fp32") as s:
@@ -151,7 +155,8 @@ def refined_measurement(p, img, steps, warmup, block, k64, fp64_ms_per_step):
         "correction_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_update": 12,
                                 "kernel": "k_sweep1<float, false, 2>"},
-        "k_rel_diff_vs_fp64": abs(k - k64) / abs(k64), "steps_compared": warmup + steps,
+        "k_rel_diff_vs_fp64_blocks_from_rest": abs(k - k64) / abs(k64) if k64 else abs(k - k64),
+        "steps_compared": warmup + steps,
         "fp32_iterations": int(n32)}
 
 
@@ -538,10 +543,10 @@ def main():
         refined = refined_measurement(p, img, args.steps, args.warmup, args.dc_block, k,
                                       elapsed / args.steps * 1e3)
         log("defect correction, blocks of %d: %.4g updates/s (%.2fx FP64), correction iteration %.3f ms "
-            "(sweep %.3f ms, %.1f %% of HBM at 12 B/update), K rel diff %.2e" % (
+            "(sweep %.3f ms, %.1f %% of HBM at 12 B/update), K rel diff %.2e (blocks from rest)" % (
                 args.dc_block, refined["value"], refined["speedup_vs_fp64"], refined["correction_iteration_ms"],
                 refined["correction_sweep_ms"], 100 * refined["correction_roofline"]["frac"],
-                refined["k_rel_diff_vs_fp64"]))
+                refined["k_rel_diff_vs_fp64_blocks_from_rest"]))
 
     wl = "spheres200_phi040_kn1_v8"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

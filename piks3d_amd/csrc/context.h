@@ -112,6 +112,12 @@ struct Ctx {
                                // workgroup ends with its wave, measured fastest)
   bool sweep_recip = false;    // fast mode: multiply by a tabulated 1 / denominator
   bool sweep_generic = false;  // single solve through the multi-right-hand-side kernel
+  int sweep_nt = 1;            // tasks per wave of the single-solve sweep (k_sweep1)
+  int oct_groups = 0;          // octant groups of the sweep (1, 2, 4, 8; 0: 4 on a context
+                               // with its own stream, 1 on a caller's stream)
+  bool oct_serial = false;     // groups one after the other instead of concurrent chains
+  hipStream_t side[7] = {};    // streams of octant groups 1.. (concurrent groups)
+  hipEvent_t fork_ev = nullptr, join_ev[7] = {};
   bool planes_fused = true;    // inlet/outlet and symmetry planes in one launch (k_planes)
   int32_t* surface = nullptr;  // Geometry::surface
   double* inv_den = nullptr;   // [first-order flags][velocity]

@@ -477,6 +477,7 @@ void build_sweep_schedule(Geometry& g, int threads) {
           }
     });
   }
+  g.hp_oct.assign(count.begin(), count.end());
   // storage windows
   g.hp_lo.assign((size_t)nb, 0);
   g.hp_len.assign((size_t)nb, 0);

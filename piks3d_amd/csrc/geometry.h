@@ -104,6 +104,9 @@ struct Geometry {
   int nhp = 0;
   std::vector<int32_t> hp_task;
   std::vector<int64_t> hp_off;
+  // hp_oct[8s + o]: first task of octant o in hyperplane s (hp_oct[8 nhp] =
+  // the task count), so that a subset of the octants can be swept alone
+  std::vector<int64_t> hp_oct;
   // Storage window of each (hyperplane, octant): every storage index its
   // tasks touch (the zero slot aside) lies in [hp_lo, hp_lo + hp_len).
   std::vector<int32_t> hp_lo, hp_len;  // nhp * 8

@@ -127,3 +127,25 @@ def test_correction_iterations_replayed_from_graphs(monkeypatch):
             runs.append((lines, s.f()))
     assert runs[0][0] == runs[1][0]
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("env", [("PIKS_SWEEP_NT", "2"), ("PIKS_SWEEP_NT", "4"), ("PIKS_OCT_GROUPS", "1"),
+                                 ("PIKS_OCT_GROUPS", "8")])
+def test_sweep_launch_shapes_keep_the_correction_bits(env, monkeypatch):
+    # the FP64 block iterations and the FP32 corrections with several sweep
+    # tasks per wave, or the octants swept in groups, give the default bits
+    p, img = _pack(n=24)
+    monkeypatch.setenv("PIKS_DC_BLOCK", "4")
+    runs = []
+    for v in (None, env[1]):
+        if v is None:
+            monkeypatch.delenv(env[0], raising=False)
+        else:
+            monkeypatch.setenv(env[0], v)
+        with Solver(p, img, precision="fp32") as s:
+            s.iterate(9)
+            lines = s.chk_converge()
+            s.set_precision("fp64")
+            runs.append((lines, s.f()))
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])

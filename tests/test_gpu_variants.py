@@ -1,6 +1,6 @@
 """The sweep's launch variants give the default path's results.
 
-A context reads nine environment settings at creation (piks3d.hip,
+A context reads twelve environment settings at creation (piks3d.hip,
 ctx_create_host; geometry.cpp): PIKS_HOST_THREADS (host threads of the image
 preprocessing, whose loops write disjoint data), PIKS_XCD_REMAP
 (XCD-contiguous task ranges), PIKS_SWEEP_BLOCK (64 or 256 threads per sweep
@@ -9,7 +9,11 @@ iteration), PIKS_SWEEP_GENERIC (the multi-right-hand-side kernel for a single
 solve), PIKS_SWEEP_TILES and PIKS_TILE (the tiled sweep, one wave per tile of
 one octant walking its planes, and its tile shape), PIKS_PLANES_FUSED (0: the
 inlet/outlet and the two symmetry directions as three launches instead of
-one) and PIKS_SWEEP_RECIP (fast
+one), PIKS_SWEEP_NT (tasks per wave of the single-solve sweep, 1-4, whose
+loads are all issued before the first update), PIKS_OCT_GROUPS (the octants
+swept in 1, 2, 4 or 8 groups, as concurrent launch chains on streams of
+their own; 4 by default) and PIKS_OCT_SERIAL (the groups one after the other on the
+context's stream) and PIKS_SWEEP_RECIP (fast
 mode: multiply by a tabulated reciprocal of the denominator).  All but the
 last compute every update with the same
 operations in the same order, so f, fw and the moments after several
@@ -36,6 +40,16 @@ VARIANTS = [
     {"PIKS_SWEEP_TILES": "1", "PIKS_TILE": "4x4x16"},
     {"PIKS_SWEEP_TILES": "1", "PIKS_TILE": "8x2x3", "PIKS_GRAPH": "1"},
     {"PIKS_PLANES_FUSED": "0"},
+    {"PIKS_SWEEP_NT": "1"},
+    {"PIKS_SWEEP_NT": "2"},
+    {"PIKS_SWEEP_NT": "3", "PIKS_XCD_REMAP": "0"},
+    {"PIKS_SWEEP_NT": "4", "PIKS_GRAPH": "1"},
+    {"PIKS_OCT_GROUPS": "1"},
+    {"PIKS_OCT_GROUPS": "2"},
+    {"PIKS_OCT_GROUPS": "4", "PIKS_SWEEP_GENERIC": "1"},
+    {"PIKS_OCT_GROUPS": "8", "PIKS_GRAPH": "1", "PIKS_SWEEP_NT": "2"},
+    {"PIKS_OCT_GROUPS": "4", "PIKS_OCT_SERIAL": "1"},
+    {"PIKS_OCT_GROUPS": "2", "PIKS_GRAPH": "1"},
 ]
 CASES = ["boxes_n2_a1_e2", "spheres40_n2"]
 STEPS = 4
@@ -44,7 +58,8 @@ STEPS = 4
 def _run(p, img, exact, env, monkeypatch):
     from piks3d_amd import Solver
     for k in ("PIKS_XCD_REMAP", "PIKS_SWEEP_BLOCK", "PIKS_GRAPH", "PIKS_SWEEP_GENERIC", "PIKS_SWEEP_RECIP",
-              "PIKS_HOST_THREADS", "PIKS_SWEEP_TILES", "PIKS_TILE", "PIKS_PLANES_FUSED"):
+              "PIKS_HOST_THREADS", "PIKS_SWEEP_TILES", "PIKS_TILE", "PIKS_PLANES_FUSED", "PIKS_SWEEP_NT",
+              "PIKS_OCT_GROUPS", "PIKS_OCT_SERIAL"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
