@@ -135,16 +135,20 @@ def refined_measurement(p, img, steps, warmup, block, k64, fp64_ms_per_step):
         t1 = time.perf_counter()
         ms = s.timing()
         m, k, err = s.chk_converge()
-        # correction iterations alone: the block's iterations 2..block
+        # correction iterations alone: the block's iterations 2..block, each
+        # timed by the wall clock around the synchronous piks_iterate (the
+        # whole iteration, defect-correction passes included) and its sweep
+        # by HIP events
         corr = []
         for q in range(block):
+            t0c = time.perf_counter()
             s.iterate(1)
-            t = s.timing()
-            corr.append(t)
+            wall = (time.perf_counter() - t0c) * 1e3
+            corr.append((wall, s.timing()[0]))
         n32 = s.get_info().steps_fp32
-    corr = sorted(corr, key=lambda t: t[3])[:-1]  # drop the block's FP64 iteration (the slowest)
-    sweep_ms = sum(t[0] for t in corr) / len(corr)
-    it_ms = sum(t[3] for t in corr) / len(corr)
+    corr = sorted(corr)[:-1]  # drop the block's FP64 iteration (the slowest)
+    sweep_ms = sum(t[1] for t in corr) / len(corr)
+    it_ms = sum(t[0] for t in corr) / len(corr)
     upd = nf * nc
     ach = upd * 12 / (sweep_ms / 1e3) / 1e9
     dt = t1 - t0
@@ -260,8 +264,9 @@ def kn_sweep_bench(args):
     (correlation length 5 voxels, porosity 0.25, percolating in x), 32 Kn
     log-spaced over [1e-3, 10], 8^3 velocities, solved as groups of right-hand
     sides sharing the geometry (one sweep launch per hyperplane for a group),
-    FP32 iterations with the FP64 finish; a few Kn are re-solved in plain FP64
-    for the same number of iterations and compared."""
+    in --precision (fp32: the FP64 defect correction with FP32 correction
+    iterations); a few Kn are re-solved in plain FP64 for the same number of
+    iterations and compared."""
     import numpy as np
     from piks3d_amd import MultiSolver, images
     from piks3d_amd.params import Params
@@ -270,13 +275,15 @@ def kn_sweep_bench(args):
                chk_converge_step=10, max_step=args.max_step, eps=args.eps, kn=1.0, press_drop=1.0, accom=1.0)
     kns = np.logspace(-3, 1, args.nkn)
     perm, steps, conv = np.zeros(len(kns)), np.zeros(len(kns), int), np.zeros(len(kns), bool)
-    updates, wall = 0.0, 0.0
+    updates, wall, solve_wall = 0.0, 0.0, 0.0
     for g0 in range(0, len(kns), args.group):
         ks = list(kns[g0:g0 + args.group])
         t = time.perf_counter()
         with MultiSolver(p, img, ks, precision=args.precision) as ms:
             nf = ms.info.nfluid
+            ts = time.perf_counter()
             r = ms.solve()
+            solve_wall += time.perf_counter() - ts
         dt = time.perf_counter() - t
         wall += dt
         updates += nf * 512 * r["steps"] * len(ks)
@@ -293,6 +300,7 @@ def kn_sweep_bench(args):
     out = {
         "metric": "multi-RHS Knudsen sweep: cell*velocity updates/s", "value": updates / wall,
         "unit": "updates/s", "higher_is_better": True, "n_gpus": 1, "wall_s": wall,
+        "solve_updates_per_s": updates / solve_wall, "solve_wall_s": solve_wall,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "kn": [float(k) for k in kns], "permeability": [float(k) for k in perm],
         "steps": [int(x) for x in steps], "converged": [bool(x) for x in conv],
@@ -306,7 +314,8 @@ def kn_sweep_bench(args):
 def large_bench(args):
     """The large single solve: a 300^3 Gaussian-random-field image
     (correlation length 6, porosity 0.20, percolating in x), 8^3 velocities,
-    Kn in {0.01, 1, 10}, FP64 and FP32 storage: sweep throughput over a few
+    Kn in {0.01, 1, 10}: plain FP64 iterations and the defect correction in
+    blocks of --dc-block (refined_measurement), throughput over a few
     iterations each."""
     from piks3d_amd import Solver, images
     from piks3d_amd.params import Params
@@ -315,19 +324,26 @@ def large_bench(args):
     for kn in (0.01, 1.0, 10.0):
         p = Params(nx=300, ny=300, nz=300, fluid_layer=4, wall_ext_order=2, nc_fundamental=4, half_range=True,
                    chk_converge_step=10, max_step=2, kn=kn, press_drop=1.0, accom=1.0)
-        for prec in ("fp64", "fp32"):
-            with Solver(p, img, precision=prec) as s:
-                s.iterate(args.warmup)
-                t = time.perf_counter()
-                s.iterate(args.steps)
-                dt = time.perf_counter() - t
-                ms = s.timing()
-                upd = s.info.nfluid * 512 * args.steps
-                rows.append({"kn": kn, "precision": prec, "updates_per_s": upd / dt,
-                             "sweep_ms_per_step": ms[0] / args.steps, "ms_per_step": dt / args.steps * 1e3,
-                             "device_gb": s.info.device_bytes / 1e9, "fluid_cells": s.info.nfluid})
-                log("Kn %g %s: %.3g updates/s, %.1f ms/step, %.1f GB" % (kn, prec, upd / dt, dt / args.steps * 1e3,
-                                                                           s.info.device_bytes / 1e9))
+        with Solver(p, img, precision="fp64") as s:
+            s.iterate(args.warmup)
+            t = time.perf_counter()
+            s.iterate(args.steps)
+            dt = time.perf_counter() - t
+            ms = s.timing()
+            k64 = s.chk_converge()[1]
+            upd = s.info.nfluid * 512 * args.steps
+            rows.append({"kn": kn, "precision": "fp64", "updates_per_s": upd / dt,
+                         "sweep_ms_per_step": ms[0] / args.steps, "ms_per_step": dt / args.steps * 1e3,
+                         "device_gb": s.info.device_bytes / 1e9, "fluid_cells": s.info.nfluid})
+            log("Kn %g fp64: %.3g updates/s, %.1f ms/step, %.1f GB" % (kn, upd / dt, dt / args.steps * 1e3,
+                                                                       s.info.device_bytes / 1e9))
+        if args.dc_block > 1:
+            r = refined_measurement(p, img, args.steps, args.warmup, args.dc_block, k64, dt / args.steps * 1e3)
+            rows.append({"kn": kn, "precision": "refined", "updates_per_s": r["value"],
+                         "ms_per_step": r["ms_per_step"], "speedup_vs_fp64": r["speedup_vs_fp64"],
+                         "correction_sweep_ms": r["correction_sweep_ms"], "block": args.dc_block})
+            log("Kn %g refined (blocks of %d): %.3g updates/s (%.2fx FP64)" % (kn, args.dc_block, r["value"],
+                                                                             r["speedup_vs_fp64"]))
     print(json.dumps({"metric": "large single solve: cell*velocity updates/s", "unit": "updates/s",
                       "value": min(r["updates_per_s"] for r in rows if r["precision"] == "fp64"),
                       "higher_is_better": True, "n_gpus": 1, "data": "synthetic", "rows": rows,
@@ -401,15 +417,17 @@ def c1_bench(args):
 def c2_solve_bench(args):
     """The 200^3 pack solved to the reference's criterion (check every 10
     iterations, eps = --eps, at most --max-step iterations) in plain FP64 and
-    as the refined solve (FP32 iterations, FP64 finish): iterations, time to
-    solution, whole-solve updates/s and the permeability difference.  With
+    as the refined solve (the FP64 defect correction, FP32 correction
+    iterations in blocks chosen from each check): iterations, the share of
+    FP32 iterations, time to solution, whole-solve updates/s and the
+    permeability difference.  With
     wallExtOrder 2 the reference's own iteration grows without bound on this
     pack after about a hundred iterations, so solves use --wall-order 1."""
     from piks3d_amd import Solver
     p, img = workload()
     p.chk_converge_step = 10
     p.wall_ext_order = args.wall_order
-    rows = {}
+    rows, hist = {}, {}
     for prec in ("fp64", "fp32"):
         with Solver(p, img, precision=prec) as s:
             nf, nc = s.info.nfluid, s.info.nc8 * 8
@@ -420,16 +438,23 @@ def c2_solve_bench(args):
                     log("C2 %s: step %d, K %.12e, error %.3e" % (prec, step, k, e))
             r = s.solve(max_step=args.max_step, eps=args.eps, callback=progress)
             dt = time.perf_counter() - t
-        rows[prec] = {"steps": r["steps"], "steps_fp32": r.get("steps32", 0), "permeability": r["permeability"],
+        hist[prec] = {h[0]: h[2] for h in r["history"]}
+        rows[prec] = {"steps": r["steps"], "steps_fp32": r.get("steps32", 0),
+                      "fp32_share": r.get("steps32", 0) / max(r["steps"], 1), "permeability": r["permeability"],
                       "error": r["error"], "solve_s": dt, "updates_per_s": nf * nc * r["steps"] / dt}
         log("C2 %s: %d steps (%d FP32), K %.12e, %.2f s" % (prec, r["steps"], r.get("steps32", 0),
                                                           r["permeability"], dt))
     a, b = rows["fp64"], rows["fp32"]
+    # K at every check both solves reached (equal iteration counts)
+    common = sorted(set(hist["fp64"]) & set(hist["fp32"]))
+    kdiff = max((abs(hist["fp32"][q] - hist["fp64"][q]) / abs(hist["fp64"][q]) for q in common
+                 if hist["fp64"][q]), default=None)
     print(json.dumps({
         "metric": "time to solution, 200^3 sphere pack, plain FP64 vs refined", "value": b["solve_s"],
         "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "rows": rows,
         "speedup_refined_vs_fp64": a["solve_s"] / b["solve_s"],
         "rel_diff_permeability": abs(a["permeability"] - b["permeability"]) / abs(a["permeability"]),
+        "max_rel_diff_permeability_at_equal_checks": kdiff, "checks_compared": len(common),
         "config": {"workload": "spheres200_phi040_kn1_v8", "eps": args.eps, "max_step": args.max_step,
                    "chk_converge_step": 10, "wall_ext_order": args.wall_order}}))
 

@@ -113,6 +113,8 @@ struct Ctx {
   bool sweep_recip = false;    // fast mode: multiply by a tabulated 1 / denominator
   bool sweep_generic = false;  // single solve through the multi-right-hand-side kernel
   int sweep_nt = 1;            // tasks per wave of the single-solve sweep (k_sweep1)
+  int multi_rb = 0;            // right-hand sides per wave of the several-Kn sweep (0: 1,
+                               // measured fastest: 1, 2, 4 took 14.1, 14.8, 16.1 s on 16 Kn)
   int oct_groups = 0;          // octant groups of the sweep (1, 2, 4, 8; 0: 4 on a context
                                // with its own stream, 1 on a caller's stream)
   bool oct_serial = false;     // groups one after the other instead of concurrent chains
@@ -151,6 +153,9 @@ struct Ctx {
   double *fw_old = nullptr, *bufe_old = nullptr, *bufw_old = nullptr;
   Mom *mom_e = nullptr, *mom_old = nullptr, *mom_r = nullptr;
   uint32_t* ns_list = nullptr;       // the non-swept slots of f (build_nonswept)
+  uint32_t* ns_act = nullptr;        // [rhs][n_ns]: the block's slots with a nonzero residual
+  uint32_t* ns_act_n = nullptr;      // [rhs][wave of k_dc_slots]: the wave's count of them
+  bool dc_active = true;             // correction iterations walk ns_act instead of ns_list
   std::vector<uint32_t> ns_host;
   int64_t n_ns = 0;
   int dc_len = 1, dc_pos = 0, dc_fixed = 0;

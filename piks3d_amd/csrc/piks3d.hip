@@ -1433,9 +1433,18 @@ __global__ void k_dc_flush_arr(TS* s, const TE* e, size_t n) {
 // storage index | octant << 28), one wave per slot and one velocity per lane;
 // old holds [right-hand side][list entry][velocity].  The swept slots take
 // these steps inside the sweep kernels (dc_previous, kDC).
-// mode 0: begin (fold, copy), 1: residual, 2: add (first = e = r)
+// mode 0: begin (fold, copy), 1: residual, 2: add (first = e = r).
+// With active lists (act_n not null; every mode launched with the same grid):
+// mode 0 also clears the slot's correction, mode 1 records each slot whose
+// residual has a nonzero bit in some velocity in the recording wave's own
+// segment of act (positions wave + j * waves, j = 0, 1, ...; its count in
+// act_n[wave], so no two waves share a counter), and mode 2 walks only those
+// slots, each wave its own segment.  For a slot whose residual is zero
+// throughout, e + r is e and the first correction's e = r is the cleared e,
+// so skipping it changes no bit.
 __global__ __launch_bounds__(256) void k_dc_slots(double* s, float* e, double* old, float* r,
-                                                  const uint32_t* list, int64_t n, Dims d, int mode, int flag) {
+                                                  const uint32_t* list, int64_t n, uint32_t* act,
+                                                  uint32_t* act_n, Dims d, int mode, int flag) {
   const int rhs = blockIdx.y;
   s += rhs * d.rs_f;
   e += rhs * d.rs_f;
@@ -1443,22 +1452,49 @@ __global__ __launch_bounds__(256) void k_dc_slots(double* s, float* e, double* o
   old += (size_t)rhs * n * d.Nc8;
   const int lane = threadIdx.x & (kWave - 1);
   const int wpb = blockDim.x / kWave;
-  for (int64_t idx = (int64_t)blockIdx.x * wpb + wave_in_block(); idx < n; idx += (int64_t)gridDim.x * wpb) {
+  const int64_t gw = (int64_t)blockIdx.x * wpb + wave_in_block();
+  const int64_t nw = (int64_t)gridDim.x * wpb;
+  const bool use_act = act_n != nullptr;
+  if (use_act) {
+    act += (size_t)rhs * n;
+    act_n += (size_t)rhs * nw;
+  }
+  if (use_act && mode == 2) {
+    const int64_t cnt = (int64_t)act_n[gw];
+    for (int64_t j = 0; j < cnt; ++j) {
+      const uint32_t w = act[gw + j * nw];
+      const size_t at = (size_t)(w >> 28) * d.ostride + (size_t)(w & 0x0fffffffu) * d.Nc8;
+      for (int l = lane; l < d.Nc8; l += kWave) e[at + l] = flag ? r[at + l] : e[at + l] + r[at + l];
+    }
+    return;
+  }
+  uint32_t rec = 0;  // slots this wave recorded (mode 1)
+  for (int64_t idx = gw; idx < n; idx += nw) {
     const uint32_t w = list[idx];
     const size_t at = (size_t)(w >> 28) * d.ostride + (size_t)(w & 0x0fffffffu) * d.Nc8;
     double* oo = old + (size_t)idx * d.Nc8;
+    bool nz = false;
     for (int l = lane; l < d.Nc8; l += kWave) {
       if (mode == 0) {
         const double v = flag ? s[at + l] + (double)e[at + l] : s[at + l];
         s[at + l] = v;
         oo[l] = v;
+        if (use_act) e[at + l] = 0.0f;
       } else if (mode == 1) {
-        r[at + l] = (float)(s[at + l] - oo[l]);
+        const float rv = (float)(s[at + l] - oo[l]);
+        r[at + l] = rv;
+        nz = nz || __builtin_bit_cast(uint32_t, rv) != 0u;
       } else {
         e[at + l] = flag ? r[at + l] : e[at + l] + r[at + l];
       }
     }
+    if (use_act && mode == 1 && __builtin_amdgcn_ballot_w64(nz) != 0) {
+      // rec counts this wave's earlier slots, so gw + rec * nw <= idx < n
+      if (lane == 0) act[gw + (int64_t)rec * nw] = w;
+      ++rec;
+    }
   }
+  if (use_act && mode == 1 && lane == 0) act_n[gw] = rec;
 }
 
 template <typename T>
@@ -1623,7 +1659,9 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
   sa.nrhs = c.nrhs;
   sa.rs_f = d.rs_f;
   sa.rs_mom = d.Ntotal;
-  const int rb = c.nrhs >= 4 ? 4 : c.nrhs >= 2 ? 2 : 1;
+  // right-hand sides per wave of the several-Kn sweep (one record shared)
+  const int rb_max = c.multi_rb > 0 ? c.multi_rb : 1;
+  const int rb = c.nrhs >= 4 && rb_max >= 4 ? 4 : c.nrhs >= 2 && rb_max >= 2 ? 2 : 1;
   const unsigned ny_sweep = (unsigned)((c.nrhs + rb - 1) / rb), R = (unsigned)c.nrhs;
   const int wpb = 4;  // waves per block for one-wave-per-item kernels
   const int timed = ev0 >= 0 ? std::max(0, std::min(nsteps, kMaxTimedSteps - ev0)) : 0;
@@ -1788,6 +1826,10 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
   return 0;
 }
 
+// Waves of every k_dc_slots launch of a context (four per block): the same
+// in every mode, as its active lists require.
+int64_t dc_slot_waves(const Ctx& c) { return 4 * (int64_t)grid_for(std::max<int64_t>(c.n_ns, 1), 4); }
+
 // Launch shape of the element-wise defect-correction kernels.
 dim3 dc_grid(size_t n) { return dim3((unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 16384))); }
 
@@ -1818,8 +1860,9 @@ int dc_arrays(Ctx& c, int mode, int flag) {
     hipLaunchKernelGGL((k_dc_add_arr<double, double>), dc_grid(nmom), dim3(256), 0, s, mom_e, (const double*)mom_r, nmom, flag);
   }
   if (c.n_ns > 0)
-    hipLaunchKernelGGL(k_dc_slots, dim3(grid_for(c.n_ns, 4), (unsigned)c.nrhs), dim3(256), 0, s, c.s64.f, c.s32.f,
-                       c.ns_old, c.r32.f, (const uint32_t*)c.ns_list, c.n_ns, c.d, mode, flag);
+    hipLaunchKernelGGL(k_dc_slots, dim3((unsigned)(dc_slot_waves(c) / 4), (unsigned)c.nrhs), dim3(256), 0, s, c.s64.f, c.s32.f,
+                       c.ns_old, c.r32.f, (const uint32_t*)c.ns_list, c.n_ns, c.ns_act,
+                       c.dc_active ? c.ns_act_n : nullptr, c.d, mode, flag);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -2005,6 +2048,12 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
     c->oct_groups = v;
   }
   if (const char* b = std::getenv("PIKS_OCT_SERIAL")) c->oct_serial = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_DC_ACTIVE")) c->dc_active = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_MULTI_RB")) {
+    const int v = std::atoi(b);
+    if (v != 1 && v != 2 && v != 4) return fail(PIKS_E_ARG, "PIKS_MULTI_RB must be 1, 2 or 4");
+    c->multi_rb = v;
+  }
   if (const char* b = std::getenv("PIKS_SWEEP_NT")) {
     const int v = std::atoi(b);
     if (v < 1 || v > 4) return fail(PIKS_E_ARG, "PIKS_SWEEP_NT must be 1..4");
@@ -2071,7 +2120,8 @@ size_t ctx_device_bytes(const Ctx& c) {
   // copies of the non-swept slots and small arrays, three moment arrays
   const size_t state32 = refined ? R * (2 * (c.n_f + c.n_fw + 2 * c.n_buf) + c.n_snap) * sizeof(float) +
                                        R * ((size_t)c.n_ns * c.d.Nc8 + c.n_fw + 2 * c.n_buf) * sizeof(double) +
-                                       R * (size_t)g.Ntotal * 3 * sizeof(Mom) + (size_t)c.n_ns * 4
+                                       R * (size_t)g.Ntotal * 3 * sizeof(Mom) + (size_t)c.n_ns * 4 +
+                                       R * ((size_t)c.n_ns + 262144) * 4
                                  : 0;
   const size_t per_cell = (size_t)g.Ntotal * (R * (sizeof(Mom) + 32 * sizeof(double)) + 1 + 4 + 4 * 4);
   const size_t walls = g.wall_cell.size() * (8 + 4 + 4 + 1 + 8 + 8) + g.wall_pos.size() * 4 +
@@ -2130,7 +2180,9 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
         (rc = dalloc(c, &c.fw_old, R * c.n_fw)) || (rc = dalloc(c, &c.bufe_old, R * c.n_buf)) ||
         (rc = dalloc(c, &c.bufw_old, R * c.n_buf)) || (rc = dalloc(c, &c.mom_e, R * (size_t)g.Ntotal)) ||
         (rc = dalloc(c, &c.mom_old, R * (size_t)g.Ntotal)) || (rc = dalloc(c, &c.mom_r, R * (size_t)g.Ntotal)) ||
-        (rc = upload(c, &c.ns_list, c.ns_host)))
+        (rc = upload(c, &c.ns_list, c.ns_host)) ||
+        (rc = dalloc(c, &c.ns_act, R * (size_t)std::max<int64_t>(c.n_ns, 1))) ||
+        (rc = dalloc(c, &c.ns_act_n, R * (size_t)dc_slot_waves(c))))
       return rc;
     c.phase = PIKS_PREC_FP32_REFINED;
   }

@@ -378,7 +378,6 @@ contains
         print*, "pressDrop = ", pressDrop
         print*, "accom = ", accom
         print*, "precision = ", trim(precision)
-        if (trim(precision) == 'fp32') print*, "refineSwitch = ", refineSwitch
         print*, "======================================"
     end subroutine print_params
 

@@ -130,10 +130,12 @@ def test_correction_iterations_replayed_from_graphs(monkeypatch):
 
 
 @pytest.mark.parametrize("env", [("PIKS_SWEEP_NT", "2"), ("PIKS_SWEEP_NT", "4"), ("PIKS_OCT_GROUPS", "1"),
-                                 ("PIKS_OCT_GROUPS", "8")])
+                                 ("PIKS_OCT_GROUPS", "8"), ("PIKS_DC_ACTIVE", "0")])
 def test_sweep_launch_shapes_keep_the_correction_bits(env, monkeypatch):
     # the FP64 block iterations and the FP32 corrections with several sweep
-    # tasks per wave, or the octants swept in groups, give the default bits
+    # tasks per wave, the octants swept in groups, or the correction's
+    # non-swept slots walked in full instead of only those with a nonzero
+    # residual, give the default bits
     p, img = _pack(n=24)
     monkeypatch.setenv("PIKS_DC_BLOCK", "4")
     runs = []
