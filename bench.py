@@ -534,10 +534,13 @@ def main():
         sync = lambda: None  # noqa: E731
 
     from piks3d_amd import Solver
+    if args.precision != "fp64":
+        sys.exit("bench.py: the benchmark line runs plain FP64 iterations and reports the refined precision "
+                 "(the defect correction) in its 'refined' object; --precision applies to --serve and --kn-sweep")
     log("generating the 200^3 sphere pack")
     p, img = workload()
     t = time.perf_counter()
-    s = Solver(p, img, precision=args.precision)
+    s = Solver(p, img, precision="fp64")
     info = s.info
     log("set-up %.1f s: Nfluid %d, walls %d, corners %d, device memory %.1f GB" % (
         time.perf_counter() - t, info.nfluid, info.nwall, info.ncorner, info.device_bytes / 1e9))
@@ -554,7 +557,7 @@ def main():
     elapsed = t1 - t0
     updates = info.nfluid * nc * args.steps
     value = updates / elapsed
-    bytes_per_update = 16 if args.precision == "fp64" else 8
+    bytes_per_update = 16
     sweep_s = ms[0] / 1e3
     achieved = updates * bytes_per_update / sweep_s / 1e9
     m, k, err = s.chk_converge()
@@ -564,7 +567,7 @@ def main():
             args.warmup + args.steps, k))
     s.close()
     refined = None
-    if args.precision == "fp64" and args.dc_block > 1:
+    if args.dc_block > 1:
         refined = refined_measurement(p, img, args.steps, args.warmup, args.dc_block, k,
                                       elapsed / args.steps * 1e3)
         log("defect correction, blocks of %d: %.4g updates/s (%.2fx FP64), correction iteration %.3f ms "
@@ -576,8 +579,8 @@ def main():
     wl = "spheres200_phi040_kn1_v8"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(wl if args.precision == "fp64" else wl + "_fp32"),
-            "kernel": "k_sweep1<double, false>" if args.precision == "fp64" else "k_sweep1<float, false>",
+            "traffic": traffic_from_profiles(wl),
+            "kernel": "k_sweep1<double, false>",
             "bytes_per_update": bytes_per_update,
             "sweep_ms_per_step": round(ms[0] / args.steps, 4)}
     cpu = None
@@ -587,12 +590,12 @@ def main():
         "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wl, "image": "200^3 overlapping spheres r~U[4,10], porosity 0.40, seed %d" % SEED,
                    "image_sha256": hashlib.sha256(img.tobytes()).hexdigest(),
                    "domain": [info.nx, info.ny, info.nz], "fluid_cells": info.nfluid,
-                   "velocities": nc, "kn": p.kn, "precision": args.precision},
+                   "velocities": nc, "kn": p.kn, "precision": "fp64"},
         "roofline": roof,
         "whole_iteration_roofline": {"achieved": round(updates * bytes_per_update / elapsed / 1e9, 1),
                                      "frac": round(updates * bytes_per_update / elapsed / 1e9 / HBM_PEAK_GBS, 4)},

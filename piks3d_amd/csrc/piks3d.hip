@@ -446,9 +446,19 @@ __global__ __launch_bounds__(kThreads) void k_sweep(SweepArgs<T> a) {
 // value.  k_sweep1 serves grids of at most 64 velocities per octant in one
 // pass of the wave; k_sweep1_passes (NT = 1) loops over passes of 64
 // velocities for larger grids.
-template <typename T, bool kRecip, bool kOnePass, int kDC = 0, int NT = 1>
+template <typename T, bool kRecip, bool kOnePass, int kDC = 0, int NT = 1, bool kMulti = false>
 __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
   static_assert(kOnePass || NT == 1, "velocity passes take one task per wave");
+  static_assert(!kMulti || (!kRecip && NT == 1), "several right-hand sides: one task per wave, no table");
+  // the right-hand side of a several-Kn context (blockIdx.y; its own state,
+  // moments, partials and mu), right-hand side 0 of a single solve
+  const int rhs = kMulti ? (int)blockIdx.y : 0;
+  T* const fr = kMulti ? a.f + (size_t)rhs * a.rs_f : a.f;
+  const Mom* const momr = kMulti ? a.mom + (size_t)rhs * a.rs_mom : a.mom;
+  double* const partr = kMulti && a.part ? a.part + (size_t)rhs * a.rs_mom * 32 : a.part;
+  float* const dcr = kMulti ? a.dc_r + (size_t)rhs * a.rs_f : a.dc_r;
+  const float* const dce = kMulti ? a.dc_e + (size_t)rhs * a.rs_f : a.dc_e;
+  const double mu0 = kMulti ? ld_const(a.mu + rhs) : a.mu0;
   // the padding tasks (t >= ntask) run the last task without storing
   // anything, so that no load waits behind the bounds test
   const uint32_t t0 = xcd_block(a.xcd_per) * (uint32_t)NT;
@@ -471,13 +481,13 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) nbo[j][q] = r.w[1 + q];
     const SweepWin win = ld_const(a.win + o[j]);
-    m[j] = ld_const(a.mom + cell[j]);
+    m[j] = ld_const(momr + cell[j]);
     cf[j] = ld_const(a.coef + (flg[j] & 7u));
-    rs[j] = __builtin_amdgcn_make_buffer_rsrc(a.f + win.off, 0, (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
+    rs[j] = __builtin_amdgcn_make_buffer_rsrc(fr + win.off, 0, (int)(win.len * (uint32_t)sizeof(T)), kBufferWord3);
     rsr[j] = rse[j] = rs[j];
     if constexpr (kDC != 0) {
-      rsr[j] = __builtin_amdgcn_make_buffer_rsrc(a.dc_r + win.off, 0, (int)(win.len * 4u), kBufferWord3);
-      rse[j] = __builtin_amdgcn_make_buffer_rsrc((float*)a.dc_e + win.off, 0, (int)(win.len * 4u), kBufferWord3);
+      rsr[j] = __builtin_amdgcn_make_buffer_rsrc(dcr + win.off, 0, (int)(win.len * 4u), kBufferWord3);
+      rse[j] = __builtin_amdgcn_make_buffer_rsrc((float*)dce + win.off, 0, (int)(win.len * 4u), kBufferWord3);
     }
   }
   const int Nc8 = a.Nc8;
@@ -505,8 +515,8 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       if (flg[j] & 8u) {  // reads a wall slot the reference switched
-        const ViewFix<T> vf = sizeof(T) == 8 && NT == 1 ? view_fix_body(a.view, cell[j], (int)o[j], ll, 0)
-                                                        : view_fix(a.view, cell[j], (int)o[j], ll, 0);
+        const ViewFix<T> vf = sizeof(T) == 8 && NT == 1 ? view_fix_body(a.view, cell[j], (int)o[j], ll, rhs)
+                                                        : view_fix(a.view, cell[j], (int)o[j], ll, rhs);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           if (vf.mask & (1u << q)) nb[j][2 + q] = vf.v[q];
@@ -523,7 +533,7 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
       tt = tt + cyl * myj;
       tt = tt + czl * mzj;
       const double feq = wl * (m[j].r + 1.0 * tt);
-      double num = a.mu0 * (feq - 0.5 * fp[j]);
+      double num = mu0 * (feq - 0.5 * fp[j]);
       num = num + cxl * cf[j].x1 * nb[j][0];
       num = num + cxl * cf[j].x2 * nb[j][1];
       num = num + cyl * cf[j].y1 * nb[j][2];
@@ -534,7 +544,7 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
       if (kRecip) {
         fnew = num * kinv[j];
       } else {
-        double den = 0.5 * a.mu0 + cxl * cf[j].dx;
+        double den = 0.5 * mu0 + cxl * cf[j].dx;
         den = den + cyl * cf[j].dy;
         den = den + czl * cf[j].dz;
         fnew = num / den;
@@ -544,7 +554,7 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
         if constexpr (kDC == 1)
           buf_store((float)(fnew - fp[j]), rsr[j], (uint32_t)ll * 4u, self[j] * 4u / (uint32_t)sizeof(T));
       }
-      if (a.part && live[j]) {
+      if (partr && live[j]) {
         const double fv = lact ? (double)(T)fnew : 0.0;
         const double z = wave_sum4(fv, cxl * fv, cyl * fv, czl * fv);
         if ((lane & 15) == 0) {
@@ -552,7 +562,7 @@ __device__ __forceinline__ void sweep1_task(const SweepArgs<T>& a) {
           // octant's: bit q of neg is set when partial q is negated
           const uint32_t q = (uint32_t)lane >> 4;
           const double zz = ((kOctNeg >> (4 * o[j] + q)) & 1u) ? -z : z;
-          double* pp = a.part + ((size_t)cell[j] * 8 + o[j]) * 4 + q;
+          double* pp = partr + ((size_t)cell[j] * 8 + o[j]) * 4 + q;
           // passes after the first add to the first pass's sum
           *pp = base == 0 ? zz : *pp + zz;
         }
@@ -567,6 +577,11 @@ __global__ __launch_bounds__(64) void k_sweep1(SweepArgs<T> a) { sweep1_task<T, 
 // the same task for grids of more than 64 velocities per octant: passes of 64
 template <typename T, bool kRecip, int kDC = 0>
 __global__ __launch_bounds__(64) void k_sweep1_passes(SweepArgs<T> a) { sweep1_task<T, kRecip, false, kDC>(a); }
+
+// the same task for one of several right-hand sides (blockIdx.y), at most 64
+// velocities per octant
+template <typename T, int kDC = 0>
+__global__ __launch_bounds__(64) void k_sweep1_rhs(SweepArgs<T> a) { sweep1_task<T, false, true, kDC, 1, true>(a); }
 
 // ------------------------------------------------------------------ tiled sweep
 
@@ -1619,6 +1634,13 @@ SweepKernel<T> sweep1_kernel(int nc8, int nt) {
 // Tasks per wave of the single-solve sweep as launched.
 inline int sweep1_tasks(const Ctx& c) { return c.d.Nc8 <= kWave ? c.sweep_nt : 1; }
 
+// Several right-hand sides, one per wave: the single-solve task kernel with
+// the right-hand side from blockIdx.y (grids of at most 64 velocities per
+// octant, one-wave workgroups), unless the generic kernel is asked for.
+inline bool multi_sweep1(const Ctx& c) {
+  return c.nrhs > 1 && c.d.Nc8 <= kWave && c.sweep_block == 64 && !c.sweep_generic;
+}
+
 // Single-solve sweep kernels of a defect-correction role.
 template <typename T, int kDC>
 SweepKernel<T> sweep_kernel_dc(const Ctx& c, int rb) {
@@ -1629,7 +1651,7 @@ SweepKernel<T> sweep_kernel_dc(const Ctx& c, int rb) {
   switch (rb) {
     case 2: return k_sweep<T, 64, false, 2, kDC>;
     case 4: return k_sweep<T, 64, false, 4, kDC>;
-    default: return k_sweep<T, 64, false, 1, kDC>;
+    default: return multi_sweep1(c) ? k_sweep1_rhs<T, kDC> : k_sweep<T, 64, false, 1, kDC>;
   }
 }
 
@@ -1756,8 +1778,9 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
         hipLaunchKernelGGL(k1, dim3((unsigned)blocks1), dim3(64), 0, sst, sa);
         continue;
       }
-      hipLaunchKernelGGL(sweep_kernel<T>(c.sweep_block, c.sweep_recip && !c.exact, rb),
-                         dim3((unsigned)blocks, ny_sweep),
+      const SweepKernel<T> km = rb == 1 && multi_sweep1(c) ? k_sweep1_rhs<T, 0>
+                                                           : sweep_kernel<T>(c.sweep_block, c.sweep_recip && !c.exact, rb);
+      hipLaunchKernelGGL(km, dim3((unsigned)blocks, ny_sweep),
                          dim3(c.sweep_block), 0, sst, sa);
     }
     if (conc) {
@@ -2133,6 +2156,11 @@ size_t ctx_device_bytes(const Ctx& c) {
          R * ((size_t)g.Ny * g.Nz * 8 + 8) + 4 * c.cx.size() * 8 + 64 * 256;
 }
 
+// Average sweep tasks (swept cell, octant) per oriented hyperplane.
+int64_t tasks_per_hyperplane(const Geometry& g) {
+  return g.nhp > 0 && !g.hp_off.empty() ? g.hp_off[(size_t)g.nhp] / g.nhp : 0;
+}
+
 int ctx_create_device(Ctx& c, hipStream_t stream) {
   HIP_TRY(hipSetDevice(c.device));
   if (stream) {
@@ -2149,11 +2177,20 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
       return fail(PIKS_E_MEM, "problem needs about " + std::to_string(need >> 20) + " MiB of device memory, " +
                                   std::to_string(free_b >> 20) + " MiB free");
   }
-  // concurrent octant groups measured fastest at 4 on the 200^3 sphere pack
-  // (19.1 ms per FP64 sweep, 19.5 with 2 groups, 20.8 with 1, 22.7 with 8:
-  // more streams than the process's hardware queues); a caller's stream (the
-  // server's slots) keeps one chain, its concurrency coming from the slots
-  if (c.oct_groups == 0) c.oct_groups = stream ? 1 : 4;
+  // Concurrent octant groups measured fastest at 4 on the 200^3 sphere pack
+  // (49 000 tasks per hyperplane on average: 19.1 ms per FP64 sweep, 19.5
+  // with 2 groups, 20.8 with 1, 22.7 with 8, more streams than the process's
+  // hardware queues).  On sphere packs of side 48-128 four chains against one
+  // took 2.05 / 1.24 ms per iteration at 3 500 tasks per hyperplane, 2.64 /
+  // 2.02 at 5 800, 4.25 / 4.37 at 12 200 and 7.66 / 8.14 at 20 900: below
+  // ~10 000 a launch's quarter does not fill the chip, so the default takes
+  // 4 from 10 000 tasks (times right-hand sides) per hyperplane on.  A
+  // caller's stream (the server's slots) keeps one chain, its concurrency
+  // coming from the slots.
+  if (c.oct_groups == 0) {
+    const int64_t per_hp = tasks_per_hyperplane(c.g) * c.nrhs;
+    c.oct_groups = !stream && per_hp >= 10000 ? 4 : 1;
+  }
   if (c.oct_groups > 1 && !c.oct_serial) {
     // default priority, like the context's stream
     HIP_TRY(hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming));
