@@ -65,23 +65,26 @@ def test_single_rhs_calls_on_a_multi_context():
     assert e_all[0] == 0.0                    # the stored mass was updated by chk_converge
 
 
-@pytest.mark.parametrize("rb", ["2", "4"])
-def test_rhs_per_wave_keeps_the_bits(rb, monkeypatch):
+@pytest.mark.parametrize("env", [{"PIKS_MULTI_RB": "2"}, {"PIKS_MULTI_RB": "4"},
+                                 {"PIKS_SWEEP_BLOCK": "256", "PIKS_MULTI_RB": "2"}, {"PIKS_SWEEP_GENERIC": "1"}],
+                         ids=lambda e: ",".join("%s=%s" % kv for kv in e.items()))
+def test_sweep_layouts_keep_the_bits(env, monkeypatch):
     # the several-Kn sweep with 2 or 4 right-hand sides per wave sharing one
-    # task record (instead of one per wave) gives the same iterates, in both
-    # precisions
+    # task record, with 256-thread workgroups, or through the generic kernel
+    # (instead of the single-solve task kernel, one right-hand side per wave)
+    # gives the same iterates, in both precisions
     z = load_golden("boxes_n2_a05_e1")
     p = params_from_golden(z)
     for prec in ("fp64", "fp32"):
         runs = []
-        for v in (None, rb):
-            if v is None:
-                monkeypatch.delenv("PIKS_MULTI_RB", raising=False)
-            else:
-                monkeypatch.setenv("PIKS_MULTI_RB", v)
+        for v in (None, env):
+            for k in ("PIKS_MULTI_RB", "PIKS_SWEEP_BLOCK", "PIKS_SWEEP_GENERIC"):
+                monkeypatch.delenv(k, raising=False)
+            for k, val in (v or {}).items():
+                monkeypatch.setenv(k, val)
             monkeypatch.setenv("PIKS_DC_BLOCK", "3")
             with MultiSolver(p, z["image"], KNS, precision=prec) as ms:
                 ms.iterate(7)
                 runs.append([np.stack(ms.macro_rhs(r)) for r in range(len(KNS))])
         for a, b in zip(*runs):
-            assert np.array_equal(a, b), (prec, rb)
+            assert np.array_equal(a, b), (prec, env)
