@@ -44,6 +44,8 @@ int set_error(int code, const std::string& msg) {
   return code;
 }
 
+int flow_check(Ctx& c);  // the single-launch sweep's failure flag (defined below)
+
 }  // namespace piks
 
 namespace {
@@ -584,6 +586,268 @@ __global__ __launch_bounds__(64) void k_sweep1_passes(SweepArgs<T> a) { sweep1_t
 // velocities per octant
 template <typename T, int kDC = 0>
 __global__ __launch_bounds__(64) void k_sweep1_rhs(SweepArgs<T> a) { sweep1_task<T, false, true, kDC, 1, true>(a); }
+
+// ------------------------------------------------------------------ single-launch sweep
+
+// The whole sweep of one source iteration in one launch: no launch gaps
+// between hyperplanes (opt-in: PIKS_SWEEP_FLOW=1).  A block sweeps the
+// octant numbered like the XCD it runs on (its hardware XCD id), so every
+// producer and consumer of an octant's slots share one L2, and takes its
+// wave number w in that octant from a per-XCD counter.  Wave w takes the
+// octant's tasks w, w + W, w + 2W, ... in hyperplane order (rec, built by
+// the host): a task's upwind neighbours come earlier in that order, so with
+// every wave of the grid resident the earliest unfinished task can always
+// proceed and no wait is circular.  That needs the GPU to itself: W is sized
+// from the occupancy, and kernels of other streams holding CUs could keep
+// waves of the grid from starting.  (Claiming tasks from a per-octant
+// counter instead, which needs no residency, measured 8-12x slower: the
+// claims serialise at 60-90 ns on the counter's address.)  A finished task
+// sets its slot's flag to the sweep's epoch after its stores have reached
+// L2; a task waits for the flags of the nearest swept upwind neighbour
+// along each axis (their own waits cover the one behind).  Flags are polled
+// with agent-scope loads, which miss the CU's L1 (an L1 line polled before
+// the flag changed would be served again: a workgroup-scope invalidate
+// leaves it).  Values are plain loads served by the XCD's L2: within the
+// launch no CU loads a slot's lines before they are final (its readers wait
+// for its flag or for the flag of a task that waited for it, and its own
+// task's read of the previous value is followed by its store, which updates
+// that CU's L1), and the L1 starts the launch invalidated.  Every wait gives
+// up after `spin` polls, and a wave beyond W on its XCD (an uneven deal of
+// blocks) stops at once: either sets err and the wave ends, so the grid
+// always drains; k_flow_done then checks that every octant's tasks were
+// done.  The update is sweep1_task's, operation for operation.
+struct FlowArgs {
+  double* f;
+  const Mom* mom;
+  double* part;             // [cell][octant][4] partial moments; null = skip
+  const TaskRec* rec;       // octant-major tasks: slot indices, bit 31 of a
+                            // neighbour word = wait for that neighbour's flag
+  uint32_t* flags;          // [octant][slot]: epoch of the last update
+  const uint32_t* epoch;    // this sweep's epoch (k_flow_epoch)
+  uint32_t* err;
+  uint32_t* wave_ctr;       // [XCD]: waves started (cleared by k_flow_epoch)
+  uint32_t* done;           // [octant]: tasks done
+  uint32_t waves;           // waves per octant
+  const double* cx;
+  const double* cy;
+  const double* cz;
+  const double* w;
+  const SweepCoef* coef;
+  const ViewArgs<double>* view;
+  double mu0;
+  int64_t off[9];           // first task of each octant in rec
+  int64_t ostride;          // elements of f per octant
+  int64_t nslot;            // flags per octant (stored slots + the zero slot)
+  uint32_t spin;            // polls before a wait gives up
+  uint32_t* diag;           // [block]: XCD id, then [grid + block]: tasks done
+};
+
+// Next epoch of the flags; the per-XCD wave and per-octant done counters
+// cleared.  st: [0] epoch, [1..6] err, [8..15] waves, [16..23] done.
+__global__ void k_flow_epoch(uint32_t* st) {
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(st, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 16) st[8 + threadIdx.x] = 0u;
+}
+
+// After the sweep: every octant's tasks done (else err, unless already set).
+struct FlowCounts {
+  uint32_t n[8];
+};
+__global__ void k_flow_done(uint32_t* st, FlowCounts c) {
+  const uint32_t o = threadIdx.x;
+  if (o < 8 && st[16 + o] != c.n[o] &&
+      __hip_atomic_fetch_add(st + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+    st[2] = o;
+    st[3] = st[16 + o];
+    st[4] = 0xffffffffu;
+    st[5] = c.n[o];
+  }
+}
+
+// XCD of every block of a grid of n one-wave blocks (flow_probe)
+__global__ void k_flow_probe(uint32_t* out, uint32_t n) {
+  if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u;
+}
+
+// What a task needs before its upwind values: its record's fields, the
+// cell's moments, the coefficients, the slot's previous value and one poll
+// of the flags it waits for (one per axis at most; an axis without a wait
+// issues no load and reads as ep).
+struct FlowTask {
+  uint32_t self, cell, flg;
+  uint32_t nbs[6];
+  const uint32_t* wt[3];
+  uint32_t v[3];
+  Mom m;
+  SweepCoef cf;
+  double fp;
+};
+
+// An agent-scope load (it misses the CU's L1); the caller makes it uniform
+// where it compares, so that several polls go out before the first returns.
+__device__ __forceinline__ uint32_t flow_poll(const uint32_t* p) {
+  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One poll of each flag the task waits for.
+__device__ __forceinline__ void flow_poll3(FlowTask& t, uint32_t ep) {
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) t.v[ax] = t.wt[ax] ? flow_poll(t.wt[ax]) : ep;
+}
+
+__device__ __forceinline__ void flow_issue(FlowTask& t, const TaskRec& r, const FlowArgs& a, const double* fo,
+                                           const uint32_t* flo, int lane, uint32_t ep) {
+  t.self = (uint32_t)r.w[0] & 0x0fffffffu;
+  t.cell = (uint32_t)r.w[7] & 0x0fffffffu;
+  t.flg = (uint32_t)r.w[7] >> 28;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) t.nbs[q] = (uint32_t)r.w[1 + q] & 0x0fffffffu;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const uint32_t w1 = (uint32_t)r.w[1 + 2 * ax], w2 = (uint32_t)r.w[2 + 2 * ax];
+    t.wt[ax] = (w1 >> 31) ? flo + t.nbs[2 * ax] : (w2 >> 31) ? flo + t.nbs[2 * ax + 1] : nullptr;
+  }
+  t.m = ld_const(a.mom + t.cell);
+  t.cf = ld_const(a.coef + (t.flg & 7u));
+  t.fp = fo[(size_t)t.self * 64 + lane];
+  flow_poll3(t, ep);
+}
+
+// Poll until the three flags read ep (false: gave up, or another wave did).
+// A wave that gives up records err = 1 and the octant, task, slot, epoch and
+// last value it saw in err[1..5] for the host's message.
+__device__ __forceinline__ bool flow_wait(FlowTask& t, uint32_t ep, uint32_t* err, uint32_t spin, uint32_t o,
+                                          uint32_t k, const uint32_t* flo) {
+  for (uint32_t it = 0; it < spin; ++it) {
+    const uint32_t v0 = __builtin_amdgcn_readfirstlane(t.v[0]), v1 = __builtin_amdgcn_readfirstlane(t.v[1]),
+                   v2 = __builtin_amdgcn_readfirstlane(t.v[2]);
+    if (v0 == ep && v1 == ep && v2 == ep) return true;
+    if ((it & 63u) == 63u && __builtin_amdgcn_readfirstlane(flow_poll(err)) != 0u) return false;
+    __builtin_amdgcn_s_sleep(2);
+    flow_poll3(t, ep);
+  }
+  if (threadIdx.x == 0) {
+    const bool b0 = t.v[0] != ep, b1 = t.v[1] != ep;
+    err[1] = o;
+    err[2] = k;
+    const uintptr_t p0 = (uintptr_t)t.wt[0], p1 = (uintptr_t)t.wt[1], p2 = (uintptr_t)t.wt[2];
+    err[3] = (uint32_t)(((b0 ? p0 : b1 ? p1 : p2) - (uintptr_t)flo) / sizeof(uint32_t));
+    err[4] = ep;
+    err[5] = b0 ? t.v[0] : b1 ? t.v[1] : t.v[2];
+    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return false;
+}
+
+// One wave: its tasks k = w, w + W, ... of its XCD's octant, pipelined: after
+// task k's stores, the loads of task k + W (flow_issue, with its first poll)
+// and the record of task k + 2W go out before the wait for those stores, so
+// the stores' acknowledgement, the next task's set-up and its first poll
+// cost one memory round trip.  Task k's flag is set before any poll of task
+// k + W is looked at again, so a wave never waits for its own task.
+// One wave: its tasks k = w, w + W, ... of its XCD's octant, pipelined:
+// after task k's stores, the loads of task k + W (flow_issue, with its first
+// poll) and the record of task k + 2W go out before the wait for those
+// stores, so the stores' acknowledgement, the next task's set-up and its
+// first poll cost one memory round trip.  Task k's flag is set before any
+// poll of task k + W is looked at again, so a wave never waits for its own
+// task.
+// Registers: 96 VGPRs (5 waves per SIMD).  Builds limited to 80 or 64 VGPRs
+// (6 or 8 waves) spill in the loop and measured 1.8x and 2.9x slower at the
+// same number of waves per octant.
+__global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a) {
+  const int lane = threadIdx.x;
+  const uint32_t o = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u);
+  uint32_t tk = 0;
+  if (lane == 0) tk = __hip_atomic_fetch_add(a.wave_ctr + o, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t wv = __builtin_amdgcn_readfirstlane(tk);
+  if (lane == 0) a.diag[blockIdx.x] = o;
+  if (wv >= a.waves) {  // more than W blocks on this XCD: its octant's waves are all there
+    if (lane == 0) {
+      a.err[1] = o;
+      a.err[2] = wv;
+      a.err[3] = 0xfffffffeu;
+      a.err[4] = a.waves;
+      __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      a.diag[gridDim.x + blockIdx.x] = 0;
+    }
+    return;
+  }
+  const int64_t base = a.off[o], n = a.off[o + 1] - base;
+  // the epoch through the vector path (an agent-scope load): a scalar load
+  // could be served by a scalar cache line of the previous kernel
+  const uint32_t ep = __builtin_amdgcn_readfirstlane(flow_poll(a.epoch));
+  double* const fo = a.f + (size_t)o * a.ostride;
+  uint32_t* const flo = a.flags + (size_t)o * a.nslot;
+  uint32_t done = 0;
+  // the lane's velocity, the same for every task of the wave (64 per octant)
+  const double cxl = a.cx[lane], cyl = a.cy[lane], czl = a.cz[lane], wl = a.w[lane];
+  const bool px = osx((int)o) > 0, py = osy((int)o) > 0, pz = osz((int)o) > 0;
+  int64_t k = wv;
+  if (k < n) {
+    FlowTask t;
+    flow_issue(t, ld_const(a.rec + base + k), a, fo, flo, lane, ep);
+    TaskRec rn = ld_const(a.rec + base + (k + a.waves < n ? k + a.waves : k));
+    while (true) {
+      if (!flow_wait(t, ep, a.err, a.spin, o, (uint32_t)k, flo)) break;
+      asm volatile("" ::: "memory");  // the upwind loads stay after the polls that cleared them
+      double nb[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) nb[q] = fo[(size_t)t.nbs[q] * 64 + lane];
+      if (t.flg & 8u) {  // reads a wall slot the reference switched
+        const ViewFix<double> vf = view_fix_body(a.view, t.cell, (int)o, lane, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (vf.mask & (1u << q)) nb[2 + q] = vf.v[q];
+      }
+      // sweep1_task's update, operation for operation
+      const double mx = px ? t.m.x : -t.m.x;
+      const double my = py ? t.m.y : -t.m.y;
+      const double mz = pz ? t.m.z : -t.m.z;
+      double tt = cxl * mx;
+      tt = tt + cyl * my;
+      tt = tt + czl * mz;
+      const double feq = wl * (t.m.r + 1.0 * tt);
+      double num = a.mu0 * (feq - 0.5 * t.fp);
+      num = num + cxl * t.cf.x1 * nb[0];
+      num = num + cxl * t.cf.x2 * nb[1];
+      num = num + cyl * t.cf.y1 * nb[2];
+      num = num + cyl * t.cf.y2 * nb[3];
+      num = num + czl * t.cf.z1 * nb[4];
+      num = num + czl * t.cf.z2 * nb[5];
+      double den = 0.5 * a.mu0 + cxl * t.cf.dx;
+      den = den + cyl * t.cf.dy;
+      den = den + czl * t.cf.dz;
+      const double fnew = num / den;
+      const uint32_t self = t.self, cell = t.cell;
+      fo[(size_t)self * 64 + lane] = fnew;
+      if (a.part) {
+        const double z = wave_sum4(fnew, cxl * fnew, cyl * fnew, czl * fnew);
+        if ((lane & 15) == 0) {
+          const uint32_t q = (uint32_t)lane >> 4;
+          const double zz = ((kOctNeg >> (4 * o + q)) & 1u) ? -z : z;
+          a.part[((size_t)cell * 8 + o) * 4 + q] = zz;
+        }
+      }
+      const int64_t kn = k + a.waves;
+      const bool more = kn < n;
+      if (more) {
+        flow_issue(t, rn, a, fo, flo, lane, ep);
+        rn = ld_const(a.rec + base + (kn + a.waves < n ? kn + a.waves : kn));
+      }
+      // the slot's values (and partials) reach L2 before its flag
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+      if (lane == 0) __hip_atomic_store(flo + self, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      ++done;
+      if (!more) break;
+      k = kn;
+    }
+  }
+  if (lane == 0) {
+    a.diag[gridDim.x + blockIdx.x] = done;
+    __hip_atomic_fetch_add(a.done + o, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
 // ------------------------------------------------------------------ tiled sweep
 
@@ -1741,7 +2005,35 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
     // joined back into the context's stream: one chain's launches fill the
     // GPU while another's drain and start.  Without them the groups run one
     // after the other.
-    const int ng = c.sweep_tiles ? 0 : std::max(1, c.oct_groups);
+    const bool flow = c.sweep_flow && sizeof(T) == sizeof(double) && im.dc == 0 && c.nrhs == 1;
+    if (flow) {
+      FlowArgs fa{};
+      fa.f = reinterpret_cast<double*>(st.f);
+      fa.mom = im.mom;
+      fa.part = sa.part;
+      fa.rec = c.flow_rec;
+      fa.flags = c.flow_flags;
+      fa.epoch = c.flow_epoch;
+      fa.err = c.flow_err;
+      fa.wave_ctr = c.flow_epoch + 8;
+      fa.done = c.flow_epoch + 16;
+      fa.waves = (uint32_t)c.flow_waves;
+      fa.cx = c.dcx; fa.cy = c.dcy; fa.cz = c.dcz; fa.w = c.dw;
+      fa.coef = c.coef;
+      fa.view = static_cast<const ViewArgs<double>*>(c.view_args[0]);
+      fa.mu0 = c.mu_r[0];
+      for (int q = 0; q < 9; ++q) fa.off[q] = c.flow_off[q];
+      fa.ostride = d.ostride;
+      fa.nslot = g.nstore + 1;
+      fa.spin = 1u << 20;
+      fa.diag = c.flow_diag;
+      FlowCounts fc{};
+      for (int q = 0; q < 8; ++q) fc.n[q] = (uint32_t)(c.flow_off[q + 1] - c.flow_off[q]);
+      hipLaunchKernelGGL(k_flow_epoch, dim3(1), dim3(64), 0, c.stream, c.flow_epoch);
+      hipLaunchKernelGGL(k_sweep_flow, dim3(8u * (unsigned)c.flow_waves), dim3(64), 0, c.stream, fa);
+      hipLaunchKernelGGL(k_flow_done, dim3(1), dim3(64), 0, c.stream, c.flow_epoch, fc);
+    }
+    const int ng = c.sweep_tiles || flow ? 0 : std::max(1, c.oct_groups);
     const bool conc = ng > 1 && c.side[0] != nullptr;
     if (conc) {
       HIP_TRY(hipEventRecord(c.fork_ev, c.stream));
@@ -1987,6 +2279,7 @@ int iterate(Ctx& c, int nsteps) {
                : enqueue_t<double>(c, c.s64, nsteps, 0, IterMode{c.mom, c.prm.press_drop, 0, 0, 0});
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c.stream));
+  if ((rc = flow_check(c))) return rc;
   const int timed = std::min(nsteps, kMaxTimedSteps);
   if (timed > 0) {
     double acc[3] = {0, 0, 0};
@@ -2078,6 +2371,12 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   }
   if (const char* b = std::getenv("PIKS_OCT_SERIAL")) c->oct_serial = std::atoi(b) != 0;
   if (const char* b = std::getenv("PIKS_DC_ACTIVE")) c->dc_active = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_SWEEP_FLOW")) c->sweep_flow = std::atoi(b) != 0;
+  if (const char* b = std::getenv("PIKS_FLOW_WAVES")) {
+    const int v = std::atoi(b);
+    if (v < 8 || v > 4096) return fail(PIKS_E_ARG, "PIKS_FLOW_WAVES must be 8..4096");
+    c->flow_waves = v;
+  }
   if (const char* b = std::getenv("PIKS_MULTI_RB")) {
     const int v = std::atoi(b);
     if (v != 1 && v != 2 && v != 4) return fail(PIKS_E_ARG, "PIKS_MULTI_RB must be 1, 2 or 4");
@@ -2160,6 +2459,133 @@ size_t ctx_device_bytes(const Ctx& c) {
   const size_t sched = c.sweep_tiles ? g.tl_rec.size() * 4 : g.hp_task.size() * 4 + g.hp_lo.size() * 16;
   return state64 + state32 + per_cell + walls + fluid + sched +
          R * ((size_t)g.Ny * g.Nz * 8 + 8) + 4 * c.cx.size() * 8 + 64 * 256;
+}
+
+// The single-launch sweep's schedule and state, refused unless the context
+// can use it: FP64 single solve, 64 velocities per octant, its own stream,
+// slot indices below 2^28, and W blocks on each of the eight XCDs (a probe
+// launch of the same grid reads each block's XCD id).  Records:
+// octant o's tasks in hyperplane order, words 1..6 the neighbours' slot
+// indices with bit 31 set on the nearest swept neighbour along each axis
+// (the one the task waits for).
+int flow_setup(Ctx& c) {
+  const Geometry& g = c.g;
+  const int64_t nc8 = c.d.Nc8, ns = g.nstore;
+  if (c.nrhs != 1 || nc8 != 64 || c.stream_ordered || c.prm.precision != PIKS_PREC_FP64 || ns + 1 >= (1 << 28))
+    return fail(PIKS_E_ARG, "PIKS_SWEEP_FLOW needs a single FP64 solve with 64 velocities per octant on a "
+                            "context with its own stream");
+  // waves per octant: all resident at once (one-wave workgroups)
+  int per_cu = 0, dev = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_flow, 64, 0));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  const int64_t resident = (int64_t)per_cu * prop.multiProcessorCount;
+  if (c.flow_waves == 0) c.flow_waves = (int)std::min<int64_t>(resident / 8, 1024);
+  if (c.flow_waves < 1 || 8 * (int64_t)c.flow_waves > resident)
+    return fail(PIKS_E_ARG, "PIKS_SWEEP_FLOW: " + std::to_string(c.flow_waves) + " waves per octant do not fit the " +
+                                std::to_string(resident) + " resident waves");
+  const unsigned nb = 8u * (unsigned)c.flow_waves;
+  {
+    uint32_t* probe = nullptr;
+    HIP_TRY(hipMalloc(&probe, nb * sizeof(uint32_t)));
+    std::vector<uint32_t> xcc(nb, 99u);
+    hipLaunchKernelGGL(k_flow_probe, dim3(nb), dim3(64), 0, c.stream, probe, nb);
+    hipError_t e = hipMemcpyAsync(xcc.data(), probe, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    (void)hipFree(probe);
+    if (e != hipSuccess) return fail(PIKS_E_HIP, std::string("flow probe: ") + hipGetErrorString(e));
+    // the sweep needs W waves on every XCD (blocks are dealt to the XCDs in
+    // turn, from a start that varies between launches)
+    std::vector<unsigned> per(16, 0);
+    for (unsigned b = 0; b < nb; ++b) ++per[xcc[b] & 15u];
+    for (unsigned x = 0; x < 8; ++x)
+      if (per[x] != (unsigned)c.flow_waves)
+        return fail(PIKS_E_HIP, "single-launch sweep: XCD " + std::to_string(x) + " ran " + std::to_string(per[x]) +
+                                    " of the probe's " + std::to_string(nb) + " blocks, not " +
+                                    std::to_string(c.flow_waves));
+  }
+  // slot index of a window-relative element offset (the zero slot: ns)
+  std::vector<uint8_t> swept((size_t)8 * (ns + 1), 0);
+  auto slot_of = [&](int64_t h, int o, uint32_t rel) -> uint32_t {
+    const int64_t lo = g.hp_lo[(size_t)h * 8 + o], len = g.hp_len[(size_t)h * 8 + o];
+    return (int64_t)rel == len * nc8 ? (uint32_t)ns : (uint32_t)(rel / nc8 + lo);
+  };
+  for (int o = 0; o < 8; ++o)
+    for (int h = 0; h < g.nhp; ++h)
+      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
+        const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
+        swept[(size_t)o * (ns + 1) + slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu)] = 1;
+      }
+  std::vector<TaskRec> rec((size_t)g.hp_off[(size_t)g.nhp]);
+  int64_t at = 0;
+  for (int o = 0; o < 8; ++o) {
+    c.flow_off[o] = at;
+    const uint8_t* sw = &swept[(size_t)o * (ns + 1)];
+    for (int h = 0; h < g.nhp; ++h)
+      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
+        const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
+        TaskRec& r = rec[(size_t)at++];
+        r.w[0] = (int32_t)(slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
+        uint32_t nbs[6];
+        for (int q = 0; q < 6; ++q) nbs[q] = slot_of(h, o, (uint32_t)w[1 + q]);
+        for (int ax = 0; ax < 3; ++ax) {
+          const uint32_t d1 = nbs[2 * ax], d2 = nbs[2 * ax + 1];
+          if (sw[d1]) nbs[2 * ax] |= 1u << 31;
+          else if (sw[d2]) nbs[2 * ax + 1] |= 1u << 31;
+        }
+        for (int q = 0; q < 6; ++q) r.w[1 + q] = (int32_t)nbs[q];
+        r.w[7] = w[7];
+      }
+  }
+  c.flow_off[8] = at;
+  int rc = 0;
+  if ((rc = upload(c, &c.flow_rec, rec)) || (rc = dalloc(c, &c.flow_flags, (size_t)8 * (ns + 1))) ||
+      (rc = dalloc(c, &c.flow_epoch, 32)) || (rc = dalloc(c, &c.flow_diag, 2 * (size_t)nb)))
+    return rc;
+  // flow_epoch: [0] the epoch, [1..6] err (flag, then the first failure),
+  // [8..15] waves started per XCD, [16..23] tasks done per octant
+  c.flow_err = c.flow_epoch + 1;
+  HIP_TRY(hipMemsetAsync(c.flow_flags, 0, (size_t)8 * (ns + 1) * sizeof(uint32_t), c.stream));
+  HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 32 * sizeof(uint32_t), c.stream));
+  HIP_TRY(hipStreamSynchronize(c.stream));  // the host vectors above end here
+  return 0;
+}
+
+// A failed wait of the single-launch sweep (its err flag), after the
+// context's stream has been synchronised: the iteration's result is not
+// valid; the context goes back to hyperplane launches.
+int flow_check(Ctx& c) {
+  if (!c.sweep_flow) return 0;
+  uint32_t err[6] = {0, 0, 0, 0, 0, 0};
+  HIP_TRY(hipMemcpy(err, c.flow_err, sizeof(err), hipMemcpyDeviceToHost));
+  if (!err[0]) return 0;
+  c.sweep_flow = false;
+  const size_t nb = 8 * (size_t)c.flow_waves;
+  std::vector<uint32_t> dg(2 * nb);
+  HIP_TRY(hipMemcpy(dg.data(), c.flow_diag, dg.size() * 4, hipMemcpyDeviceToHost));
+  // diag: block b swept octant dg[b] and did dg[nb + b] tasks
+  std::vector<uint64_t> per(8, 0), done(8, 0);
+  for (size_t b = 0; b < nb; ++b) {
+    ++per[dg[b] & 7u];
+    done[dg[b] & 7u] += dg[nb + b];
+  }
+  std::string blocks, tasks;
+  for (int o = 0; o < 8; ++o) {
+    blocks += " " + std::to_string(per[(size_t)o]);
+    tasks += " " + std::to_string(done[(size_t)o]) + "/" + std::to_string(c.flow_off[o + 1] - c.flow_off[o]);
+  }
+  const std::string what =
+      err[3] == 0xffffffffu   ? "octant " + std::to_string(err[1]) + " did " + std::to_string(err[2]) + " of its " +
+                                  std::to_string(err[4]) + " tasks"
+      : err[3] == 0xfffffffeu ? "XCD " + std::to_string(err[1]) + " started wave " + std::to_string(err[2]) +
+                                    " of " + std::to_string(err[4])
+                              : "a dependency wait gave up (octant " + std::to_string(err[1]) + ", task " + std::to_string(err[2]) +
+                ", slot " + std::to_string(err[3]) + ", epoch " + std::to_string(err[4]) + ", flag seen " +
+                std::to_string(err[5]) + ")";
+  return fail(PIKS_E_HIP, "single-launch sweep: " + what + "; blocks per octant" + blocks + "; tasks done" + tasks +
+                              "; the iteration's result is not valid and the context uses hyperplane launches "
+                              "from now on");
 }
 
 // Average sweep tasks (swept cell, octant) per oriented hyperplane.
@@ -2284,6 +2710,7 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
       win[q].pad = 0;
     }
     if ((rc = upload(c, &c.hp_win, win))) return rc;
+    if (c.sweep_flow && (rc = flow_setup(c))) return rc;
   }
   for (int F = 0; F < 4; ++F) {
     if ((rc = upload(c, &c.pos[F], g.pos[F]))) return rc;
@@ -2478,7 +2905,7 @@ int piks_iterate(void* ctx, int32_t nsteps) {
   if (c.use_graph) {
     if (int rc = ctx_enqueue(c, nsteps)) return rc;
     HIP_TRY(hipStreamSynchronize(c.stream));
-    return 0;
+    return flow_check(c);
   }
   return iterate(c, nsteps);
 }
