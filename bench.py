@@ -425,6 +425,52 @@ def c1_bench(args):
                    "eps": p.eps, "chk_converge_step": p.chk_converge_step}}))
 
 
+def flow_compare_bench(args):
+    """The single-launch sweep (PIKS_SWEEP_FLOW=1) beside the default
+    hyperplane launches on sphere packs of the headline workload's kind
+    (porosity 0.40, radii 4-10, seed 2024, 4 fluid layers) at the sizes of
+    --flow-sizes: sweep device time per iteration (mean over --steps
+    iterations after one untimed one) and whether the two paths leave the
+    same state (f compared bit for bit up to 100^3, above that the moments
+    and the check), in FP64."""
+    import numpy as np
+    from piks3d_amd import Solver, images
+    from piks3d_amd.params import Params
+    rows = []
+    for n in [int(v) for v in args.flow_sizes.split(",")]:
+        img = images.sphere_pack(n, 0.40, 4.0, 10.0, seed=SEED)
+        p = Params(nx=n, ny=n, nz=n, fluid_layer=4, wall_ext_order=2, ref_l=1.0, nc_fundamental=4,
+                   half_range=True, chk_converge_step=10, max_step=2, eps=0.0, kn=1.0, press_drop=1.0,
+                   accom=1.0)
+        res = {}
+        for mode in ("hyperplanes", "single_launch"):
+            if mode == "single_launch":
+                os.environ["PIKS_SWEEP_FLOW"] = "1"
+            else:
+                os.environ.pop("PIKS_SWEEP_FLOW", None)
+            with Solver(p, img, precision="fp64") as s:
+                s.iterate(1)
+                t = time.perf_counter()
+                s.iterate(args.steps)
+                wall = time.perf_counter() - t
+                sweep_ms = s.timing()[0] / args.steps
+                chk = s.chk_converge()
+                state = s.f() if n <= 100 else np.stack(s.macro())
+                nf = s.info.nfluid
+            res[mode] = (sweep_ms, wall / args.steps * 1e3, chk, state)
+            log("%d^3 %s: sweep %.3f ms/iteration, iteration %.3f ms" % (n, mode, sweep_ms,
+                                                                        wall / args.steps * 1e3))
+        os.environ.pop("PIKS_SWEEP_FLOW", None)
+        h, f = res["hyperplanes"], res["single_launch"]
+        rows.append({"n": n, "fluid_cells": nf, "sweep_ms_hyperplanes": h[0], "sweep_ms_single_launch": f[0],
+                     "iteration_ms_hyperplanes": h[1], "iteration_ms_single_launch": f[1],
+                     "speedup": h[0] / f[0], "same_bits": bool(h[2] == f[2] and np.array_equal(h[3], f[3])),
+                     "compared": "f" if n <= 100 else "moments and check"})
+    print(json.dumps({"metric": "sweep device time per iteration, single launch vs hyperplane launches",
+                      "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+                      "rows": rows, "config": {"workload": "spheres_phi040_r4-10_kn1_v8", "steps": args.steps}}))
+
+
 def c2_solve_bench(args):
     """The 200^3 pack solved to the reference's criterion (check every 10
     iterations, eps = --eps, at most --max-step iterations) in plain FP64 and
@@ -494,6 +540,9 @@ def main():
                     help="--c2-solve: wallExtOrder (the reference's iteration diverges on this pack with 2)")
     ap.add_argument("--c1", action="store_true",
                     help="the 64^3 sphere configuration to convergence, GPU and reference program")
+    ap.add_argument("--flow-compare", action="store_true",
+                    help="single-launch sweep beside the hyperplane launches at --flow-sizes")
+    ap.add_argument("--flow-sizes", default="40,64,100,200")
     ap.add_argument("--large", action="store_true",
                     help="run the 300^3 random-field single solve at Kn 0.01, 1, 10 instead")
     ap.add_argument("--nkn", type=int, default=32)
@@ -533,6 +582,9 @@ def main():
         return
     if args.c1:
         c1_bench(args)
+        return
+    if args.flow_compare:
+        flow_compare_bench(args)
         return
     if args.c2_solve:
         c2_solve_bench(args)
