@@ -430,7 +430,7 @@ def flow_compare_bench(args):
     hyperplane launches on sphere packs of the headline workload's kind
     (porosity 0.40, radii 4-10, seed 2024, 4 fluid layers) at the sizes of
     --flow-sizes: sweep device time per iteration (mean over --steps
-    iterations after one untimed one) and whether the two paths leave the
+    iterations after --warmup untimed ones) and whether the two paths leave the
     same state (f compared bit for bit up to 100^3, above that the moments
     and the check), in FP64."""
     import numpy as np
@@ -449,7 +449,7 @@ def flow_compare_bench(args):
             else:
                 os.environ.pop("PIKS_SWEEP_FLOW", None)
             with Solver(p, img, precision="fp64") as s:
-                s.iterate(1)
+                s.iterate(max(1, args.warmup))
                 t = time.perf_counter()
                 s.iterate(args.steps)
                 wall = time.perf_counter() - t
@@ -468,7 +468,8 @@ def flow_compare_bench(args):
                      "compared": "f" if n <= 100 else "moments and check"})
     print(json.dumps({"metric": "sweep device time per iteration, single launch vs hyperplane launches",
                       "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
-                      "rows": rows, "config": {"workload": "spheres_phi040_r4-10_kn1_v8", "steps": args.steps}}))
+                      "rows": rows, "config": {"workload": "spheres_phi040_r4-10_kn1_v8", "steps": args.steps,
+                                                         "warmup": max(1, args.warmup)}}))
 
 
 def c2_solve_bench(args):

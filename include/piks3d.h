@@ -87,7 +87,8 @@ typedef struct piks_info {
     int32_t precision;
     int64_t steps_fp32;            /* FP32 correction iterations run (PIKS_PREC_FP32_REFINED) */
     int32_t dc_block;              /* current defect-correction block length (1: plain FP64) */
-    int32_t reserved;
+    int32_t sweep_path;            /* FP64 sweep: 0 one launch per hyperplane, 1 tiles, 2 the
+                                      whole sweep in one launch (no counterpart in the reference) */
 } piks_info;
 
 /* Set-up: replaces setupVelocityGrid + setupPhysicalGrid + setupFlow +

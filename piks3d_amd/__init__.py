@@ -54,7 +54,7 @@ class PiksInfo(ctypes.Structure):
         ("ds", ctypes.c_double), ("mu", ctypes.c_double), ("diff_flux", ctypes.c_double),
         ("mass", ctypes.c_double), ("device_bytes", ctypes.c_int64),
         ("coupled_walls", ctypes.c_int32), ("precision", ctypes.c_int32),
-        ("steps_fp32", ctypes.c_int64), ("dc_block", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("steps_fp32", ctypes.c_int64), ("dc_block", ctypes.c_int32), ("sweep_path", ctypes.c_int32),
     ]
 
 

@@ -115,15 +115,18 @@ struct Ctx {
   int sweep_nt = 1;            // tasks per wave of the single-solve sweep (k_sweep1)
   // single-launch sweep (k_sweep_flow; FP64 iterations of a single solve with
   // 64 velocities per octant on a context with its own stream): octant o on
-  // XCD o, tasks statically assigned to flow_waves waves per octant
-  bool sweep_flow = false;
+  // XCD o, tasks split statically over flow_waves waves per octant, or claimed
+  // from a counter when the octant's waves are not all resident
+  int flow_req = -1;                  // -1 where it applies, 0 off, 1 required
+  bool sweep_flow = false;            // in use
   int flow_waves = 0;
-  TaskRec* flow_rec = nullptr;        // octant-major tasks with slot indices and wait bits
+  int flow_barrier = 1 << 12;         // polls for an octant's waves to arrive
+  TaskRec* flow_rec = nullptr;        // octant-major tasks (flow_setup)
   int64_t flow_off[9] = {};
   uint32_t* flow_flags = nullptr;     // [octant][slot]
-  uint32_t* flow_epoch = nullptr;
+  uint32_t* flow_epoch = nullptr;     // the launch's state (k_flow_epoch)
   uint32_t* flow_err = nullptr;
-  uint32_t* flow_diag = nullptr;      // per block: XCD id and tasks done (the failure message)
+  uint32_t* flow_diag = nullptr;      // per block: octant and tasks done (the failure message)
   int multi_rb = 0;            // right-hand sides per wave of the several-Kn sweep (0: 1,
                                // measured fastest: 1, 2, 4 took 14.1, 14.8, 16.1 s on 16 Kn)
   int oct_groups = 0;          // octant groups of the sweep (1, 2, 4, 8; 0: 4 on a context

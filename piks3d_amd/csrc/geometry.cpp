@@ -714,4 +714,15 @@ void build_wall_positions(Geometry& g, int threads) {
   });
 }
 
+int64_t flow_switched_word(const Geometry& g, int o, int64_t n, int axis) {
+  if (g.lab[(size_t)n] < 20) return -1;
+  const int w = g.wid[(size_t)n];
+  const int src = (int)((g.wall_views[(size_t)w] >> (8 * o + (axis == 2 ? 4 : 0))) & 15u);
+  if (src == 0) return -1;
+  if (src <= 8) return (int64_t)(uint32_t)g.pos[ofam(src - 1)][(size_t)n] | (int64_t)((src - 1) ^ o) << 28;
+  const int64_t corner = g.wall_corner[(size_t)w];
+  if (corner < 0 || corner * 8 + o >= (1 << 28)) return -2;
+  return (corner * 8 + o) | 0x90000000ll;
+}
+
 }  // namespace piks

@@ -161,6 +161,15 @@ constexpr int tile_rec_xhalo(int ns, int q) { return 1 + ns + q; }
 constexpr int tile_rec_yhalo(int ns, int ty, int q) { return 1 + ns + 2 * ty + q; }
 constexpr int tile_rec_cell(int ns, int tx, int ty, int s) { return 1 + ns + 2 * ty + 2 * tx + s; }
 
+// Source word of a single-launch sweep record for the read of neighbour n
+// (a y neighbour for axis 1, z for axis 2) by a task of octant o, when the
+// reference switched that wall slot (the views of Geometry::wall_views, read
+// as the sweep's switched_source reads them): bits 0..27 an index, 28..30 x,
+// bit 31 c; c = 0: storage index of octant o ^ x (the switched octant's own
+// slot of n), c = 1 and x = 1: corner store entry corner * 8 + o.  Returns
+// -1 when the plain slot applies, -2 for a corner view without a corner.
+int64_t flow_switched_word(const Geometry& g, int o, int64_t n, int axis);
+
 // The stored slots (octant o, storage index p) that octant o's sweep does not
 // update (walls, ghost planes, inflow layers, ...), as p | o << 28 sorted by
 // octant and index: the slots a defect-correction iteration updates outside

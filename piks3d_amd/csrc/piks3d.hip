@@ -620,33 +620,39 @@ struct FlowArgs {
   double* f;
   const Mom* mom;
   double* part;             // [cell][octant][4] partial moments; null = skip
-  const TaskRec* rec;       // octant-major tasks: slot indices, bit 31 of a
-                            // neighbour word = wait for that neighbour's flag
+  const TaskRec* rec;       // octant-major tasks (flow_setup): word 0 the slot,
+                            // words 1..6 the upwind sources (flow_source)
   uint32_t* flags;          // [octant][slot]: epoch of the last update
   const uint32_t* epoch;    // this sweep's epoch (k_flow_epoch)
   uint32_t* err;
   uint32_t* wave_ctr;       // [XCD]: waves started (cleared by k_flow_epoch)
   uint32_t* done;           // [octant]: tasks done
+  uint32_t* mode;           // [octant]: 0 undecided, 1 static split, 2 claims
+  uint32_t* next;           // [octant]: tasks claimed (claims mode)
   uint32_t waves;           // waves per octant
+  uint32_t barrier;         // polls a wave waits for its octant's W waves to arrive
   const double* cx;
   const double* cy;
   const double* cz;
   const double* w;
   const SweepCoef* coef;
-  const ViewArgs<double>* view;
+  const double* fw;         // corner store [corner][octant][velocity]
   double mu0;
   int64_t off[9];           // first task of each octant in rec
   int64_t ostride;          // elements of f per octant
   int64_t nslot;            // flags per octant (stored slots + the zero slot)
   uint32_t spin;            // polls before a wait gives up
+  uint32_t probe;           // nonzero: residency probe only (flow_setup), this many polls
   uint32_t* diag;           // [block]: XCD id, then [grid + block]: tasks done
 };
 
-// Next epoch of the flags; the per-XCD wave and per-octant done counters
-// cleared.  st: [0] epoch, [1..6] err, [8..15] waves, [16..23] done.
+// Next epoch of the flags; the counters of the launch cleared.  st: [0]
+// epoch, [1..6] err, [8..15] waves started per XCD, [16..23] tasks done per
+// octant, [24] the residency probe's count, [32..39] mode per octant,
+// [40..47] tasks claimed per octant.
 __global__ void k_flow_epoch(uint32_t* st) {
   if (threadIdx.x == 0) __hip_atomic_fetch_add(st, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (threadIdx.x < 16) st[8 + threadIdx.x] = 0u;
+  if (threadIdx.x < 56) st[8 + threadIdx.x] = 0u;
 }
 
 // After the sweep: every octant's tasks done (else err, unless already set).
@@ -670,16 +676,15 @@ __global__ void k_flow_probe(uint32_t* out, uint32_t n) {
 }
 
 // What a task needs before its upwind values: its record's fields, the
-// cell's moments, the coefficients, the slot's previous value and one poll
-// of the flags it waits for (one per axis at most; an axis without a wait
+// cell's moments, the slot's previous value and one poll of the flags it
+// waits for (one per axis at most; an axis without a wait
 // issues no load and reads as ep).
 struct FlowTask {
   uint32_t self, cell, flg;
-  uint32_t nbs[6];
+  uint32_t nbs[6];          // the record's source words
   const uint32_t* wt[3];
   uint32_t v[3];
   Mom m;
-  SweepCoef cf;
   double fp;
 };
 
@@ -695,22 +700,52 @@ __device__ __forceinline__ void flow_poll3(FlowTask& t, uint32_t ep) {
   for (int ax = 0; ax < 3; ++ax) t.v[ax] = t.wt[ax] ? flow_poll(t.wt[ax]) : ep;
 }
 
-__device__ __forceinline__ void flow_issue(FlowTask& t, const TaskRec& r, const FlowArgs& a, const double* fo,
-                                           const uint32_t* flo, int lane, uint32_t ep) {
+__device__ __forceinline__ void flow_issue(FlowTask& t, const TaskRec& r, const Mom* __restrict__ mom,
+                                           const double* fo, const uint32_t* flo, int lane, uint32_t ep) {
   t.self = (uint32_t)r.w[0] & 0x0fffffffu;
   t.cell = (uint32_t)r.w[7] & 0x0fffffffu;
   t.flg = (uint32_t)r.w[7] >> 28;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) t.nbs[q] = (uint32_t)r.w[1 + q] & 0x0fffffffu;
+  for (int q = 0; q < 6; ++q) t.nbs[q] = (uint32_t)r.w[1 + q];
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
-    const uint32_t w1 = (uint32_t)r.w[1 + 2 * ax], w2 = (uint32_t)r.w[2 + 2 * ax];
-    t.wt[ax] = (w1 >> 31) ? flo + t.nbs[2 * ax] : (w2 >> 31) ? flo + t.nbs[2 * ax + 1] : nullptr;
+    const uint32_t w1 = t.nbs[2 * ax], w2 = t.nbs[2 * ax + 1];
+    t.wt[ax] = (w1 >> 28) == 8u   ? flo + (w1 & 0x0fffffffu)
+               : (w2 >> 28) == 8u ? flo + (w2 & 0x0fffffffu)
+                                  : nullptr;
   }
-  t.m = ld_const(a.mom + t.cell);
-  t.cf = ld_const(a.coef + (t.flg & 7u));
+  t.m = ld_const(mom + t.cell);
   t.fp = fo[(size_t)t.self * 64 + lane];
   flow_poll3(t, ep);
+}
+
+// The first velocity of an upwind source word of a flow record: bits 0..27
+// an index, bits 28..30 x, bit 31 c.  c = 0: slot of octant o ^ x; c = 1,
+// x = 0: slot of octant o, whose flag the task waits for; c = 1, x != 0:
+// corner store entry (corner * 8 + octant).  The host resolves the reads
+// of wall slots the reference switched (switched_source) into the second and
+// third forms, so the sweep reads every source through this one path.
+__device__ __forceinline__ const double* flow_source(const FlowArgs& a, uint32_t o, uint32_t wq) {
+  const uint32_t x = (wq >> 28) & 7u;
+  const size_t idx = (size_t)(wq & 0x0fffffffu) * 64;
+  return (wq >> 31) && x ? a.fw + idx : a.f + (size_t)(o ^ x) * a.ostride + idx;
+}
+
+// Claim the octant's next task (lane 0's value; the caller makes it uniform
+// where it uses it, so that the claim overlaps other loads).
+__device__ __forceinline__ uint32_t flow_claim(uint32_t* next, int lane) {
+  uint32_t v = 0;
+  if (lane == 0) v = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return v;
+}
+
+// Set the octant's mode to m unless it is set: the mode that holds.
+__device__ __forceinline__ uint32_t flow_decide(uint32_t* mode, uint32_t m, int lane) {
+  uint32_t old = 0;
+  if (lane == 0)
+    __hip_atomic_compare_exchange_strong(mode, &old, m, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  old = __builtin_amdgcn_readfirstlane(old);
+  return old ? old : m;
 }
 
 // Poll until the three flags read ep (false: gave up, or another wave did).
@@ -752,27 +787,38 @@ __device__ __forceinline__ bool flow_wait(FlowTask& t, uint32_t ep, uint32_t* er
 // first poll cost one memory round trip.  Task k's flag is set before any
 // poll of task k + W is looked at again, so a wave never waits for its own
 // task.
-// Registers: 96 VGPRs (5 waves per SIMD).  Builds limited to 80 or 64 VGPRs
-// (6 or 8 waves) spill in the loop and measured 1.8x and 2.9x slower at the
-// same number of waves per octant.
-__global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a) {
+// The records, moments and coefficients, which nothing in the launch writes,
+// come through restrict arguments: their loads are then scalar loads and the
+// kernel needs 61 VGPRs (8 waves per SIMD; 96 with them in vector registers
+// and the switched reads resolved on the device).
+__global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __restrict__ rec,
+                                                   const Mom* __restrict__ mom, const SweepCoef* __restrict__ coef) {
   const int lane = threadIdx.x;
+  if (a.probe) {  // are all blocks of this grid resident at once?  (st[24] counts them in)
+    if (lane == 0) __hip_atomic_fetch_add(a.done + 8, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t seen = 0;
+    for (uint32_t it = 0; it < a.probe; ++it) {
+      seen = __builtin_amdgcn_readfirstlane(flow_poll(a.done + 8));
+      if (seen >= gridDim.x) break;
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (lane == 0) a.diag[blockIdx.x] = seen >= gridDim.x ? 0xffffffffu : seen;
+    return;
+  }
   const uint32_t o = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u);
   uint32_t tk = 0;
   if (lane == 0) tk = __hip_atomic_fetch_add(a.wave_ctr + o, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t wv = __builtin_amdgcn_readfirstlane(tk);
   if (lane == 0) a.diag[blockIdx.x] = o;
-  if (wv >= a.waves) {  // more than W blocks on this XCD: its octant's waves are all there
-    if (lane == 0) {
-      a.err[1] = o;
-      a.err[2] = wv;
-      a.err[3] = 0xfffffffeu;
-      a.err[4] = a.waves;
-      __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      a.diag[gridDim.x + blockIdx.x] = 0;
-    }
-    return;
+  // the octant's mode: the static split once its W waves have all arrived
+  // (the W-th arrival sets it), claims if a wave waited `barrier` polls first
+  uint32_t md = wv + 1 == a.waves ? flow_decide(a.mode + o, 1u, lane) : 0u;
+  for (uint32_t it = 0; md == 0u && it < a.barrier; ++it) {
+    md = __builtin_amdgcn_readfirstlane(flow_poll(a.mode + o));
+    if (md == 0u) __builtin_amdgcn_s_sleep(2);
   }
+  if (md == 0u) md = flow_decide(a.mode + o, 2u, lane);
+  const bool claims = md == 2u;
   const int64_t base = a.off[o], n = a.off[o + 1] - base;
   // the epoch through the vector path (an agent-scope load): a scalar load
   // could be served by a scalar cache line of the previous kernel
@@ -783,23 +829,23 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a) {
   // the lane's velocity, the same for every task of the wave (64 per octant)
   const double cxl = a.cx[lane], cyl = a.cy[lane], czl = a.cz[lane], wl = a.w[lane];
   const bool px = osx((int)o) > 0, py = osy((int)o) > 0, pz = osz((int)o) > 0;
-  int64_t k = wv;
+  uint32_t* const next = a.next + o;
+  // static: tasks wv, wv + W, ... (a wave beyond W has none); claims: the
+  // next unclaimed task, claimed one task ahead
+  int64_t k = claims ? (int64_t)__builtin_amdgcn_readfirstlane(flow_claim(next, lane)) : wv < a.waves ? wv : n;
   if (k < n) {
     FlowTask t;
-    flow_issue(t, ld_const(a.rec + base + k), a, fo, flo, lane, ep);
-    TaskRec rn = ld_const(a.rec + base + (k + a.waves < n ? k + a.waves : k));
+    flow_issue(t, ld_const(rec + base + k), mom, fo, flo, lane, ep);
+    int64_t kn = claims ? (int64_t)__builtin_amdgcn_readfirstlane(flow_claim(next, lane)) : k + a.waves;
+    TaskRec rn = ld_const(rec + base + (kn < n ? kn : k));
     while (true) {
       if (!flow_wait(t, ep, a.err, a.spin, o, (uint32_t)k, flo)) break;
       asm volatile("" ::: "memory");  // the upwind loads stay after the polls that cleared them
       double nb[6];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) nb[q] = fo[(size_t)t.nbs[q] * 64 + lane];
-      if (t.flg & 8u) {  // reads a wall slot the reference switched
-        const ViewFix<double> vf = view_fix_body(a.view, t.cell, (int)o, lane, 0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (vf.mask & (1u << q)) nb[2 + q] = vf.v[q];
-      }
+      for (int q = 0; q < 6; ++q) nb[q] = flow_source(a, o, t.nbs[q])[lane];
+      // the coefficients (eight entries, cache-resident) with the upwind values
+      const SweepCoef cf = ld_const(coef + (t.flg & 7u));
       // sweep1_task's update, operation for operation
       const double mx = px ? t.m.x : -t.m.x;
       const double my = py ? t.m.y : -t.m.y;
@@ -809,15 +855,15 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a) {
       tt = tt + czl * mz;
       const double feq = wl * (t.m.r + 1.0 * tt);
       double num = a.mu0 * (feq - 0.5 * t.fp);
-      num = num + cxl * t.cf.x1 * nb[0];
-      num = num + cxl * t.cf.x2 * nb[1];
-      num = num + cyl * t.cf.y1 * nb[2];
-      num = num + cyl * t.cf.y2 * nb[3];
-      num = num + czl * t.cf.z1 * nb[4];
-      num = num + czl * t.cf.z2 * nb[5];
-      double den = 0.5 * a.mu0 + cxl * t.cf.dx;
-      den = den + cyl * t.cf.dy;
-      den = den + czl * t.cf.dz;
+      num = num + cxl * cf.x1 * nb[0];
+      num = num + cxl * cf.x2 * nb[1];
+      num = num + cyl * cf.y1 * nb[2];
+      num = num + cyl * cf.y2 * nb[3];
+      num = num + czl * cf.z1 * nb[4];
+      num = num + czl * cf.z2 * nb[5];
+      double den = 0.5 * a.mu0 + cxl * cf.dx;
+      den = den + cyl * cf.dy;
+      den = den + czl * cf.dz;
       const double fnew = num / den;
       const uint32_t self = t.self, cell = t.cell;
       fo[(size_t)self * 64 + lane] = fnew;
@@ -829,11 +875,11 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a) {
           a.part[((size_t)cell * 8 + o) * 4 + q] = zz;
         }
       }
-      const int64_t kn = k + a.waves;
       const bool more = kn < n;
+      uint32_t knn = 0;
       if (more) {
-        flow_issue(t, rn, a, fo, flo, lane, ep);
-        rn = ld_const(a.rec + base + (kn + a.waves < n ? kn + a.waves : kn));
+        flow_issue(t, rn, mom, fo, flo, lane, ep);
+        if (claims) knn = flow_claim(next, lane);
       }
       // the slot's values (and partials) reach L2 before its flag
       __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -841,6 +887,8 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a) {
       ++done;
       if (!more) break;
       k = kn;
+      kn = claims ? (int64_t)__builtin_amdgcn_readfirstlane(knn) : kn + a.waves;
+      rn = ld_const(rec + base + (kn < n ? kn : k));
     }
   }
   if (lane == 0) {
@@ -2017,10 +2065,13 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
       fa.err = c.flow_err;
       fa.wave_ctr = c.flow_epoch + 8;
       fa.done = c.flow_epoch + 16;
+      fa.mode = c.flow_epoch + 32;
+      fa.next = c.flow_epoch + 40;
       fa.waves = (uint32_t)c.flow_waves;
+      fa.barrier = (uint32_t)c.flow_barrier;
       fa.cx = c.dcx; fa.cy = c.dcy; fa.cz = c.dcz; fa.w = c.dw;
       fa.coef = c.coef;
-      fa.view = static_cast<const ViewArgs<double>*>(c.view_args[0]);
+      fa.fw = reinterpret_cast<const double*>(st.fw);
       fa.mu0 = c.mu_r[0];
       for (int q = 0; q < 9; ++q) fa.off[q] = c.flow_off[q];
       fa.ostride = d.ostride;
@@ -2030,7 +2081,8 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
       FlowCounts fc{};
       for (int q = 0; q < 8; ++q) fc.n[q] = (uint32_t)(c.flow_off[q + 1] - c.flow_off[q]);
       hipLaunchKernelGGL(k_flow_epoch, dim3(1), dim3(64), 0, c.stream, c.flow_epoch);
-      hipLaunchKernelGGL(k_sweep_flow, dim3(8u * (unsigned)c.flow_waves), dim3(64), 0, c.stream, fa);
+      hipLaunchKernelGGL(k_sweep_flow, dim3(8u * (unsigned)c.flow_waves), dim3(64), 0, c.stream, fa, fa.rec,
+                         fa.mom, fa.coef);
       hipLaunchKernelGGL(k_flow_done, dim3(1), dim3(64), 0, c.stream, c.flow_epoch, fc);
     }
     const int ng = c.sweep_tiles || flow ? 0 : std::max(1, c.oct_groups);
@@ -2371,12 +2423,21 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   }
   if (const char* b = std::getenv("PIKS_OCT_SERIAL")) c->oct_serial = std::atoi(b) != 0;
   if (const char* b = std::getenv("PIKS_DC_ACTIVE")) c->dc_active = std::atoi(b) != 0;
-  if (const char* b = std::getenv("PIKS_SWEEP_FLOW")) c->sweep_flow = std::atoi(b) != 0;
+  // the single-launch sweep: on where it applies unless a setting that
+  // shapes the hyperplane sweep asks for that path; PIKS_SWEEP_FLOW=0 turns
+  // it off, =1 makes a context it cannot serve an error
+  const char* const shaping[] = {"PIKS_XCD_REMAP", "PIKS_SWEEP_BLOCK", "PIKS_SWEEP_TILES", "PIKS_SWEEP_GENERIC",
+                                 "PIKS_OCT_GROUPS", "PIKS_OCT_SERIAL", "PIKS_SWEEP_NT", "PIKS_SWEEP_RECIP"};
+  c->flow_req = p->sweep != 0 ? 0 : -1;
+  for (const char* v : shaping)
+    if (std::getenv(v)) c->flow_req = 0;
+  if (const char* b = std::getenv("PIKS_SWEEP_FLOW")) c->flow_req = std::atoi(b) != 0 ? 1 : 0;
   if (const char* b = std::getenv("PIKS_FLOW_WAVES")) {
     const int v = std::atoi(b);
     if (v < 8 || v > 4096) return fail(PIKS_E_ARG, "PIKS_FLOW_WAVES must be 8..4096");
     c->flow_waves = v;
   }
+  if (const char* b = std::getenv("PIKS_FLOW_BARRIER")) c->flow_barrier = std::atoi(b);
   if (const char* b = std::getenv("PIKS_MULTI_RB")) {
     const int v = std::atoi(b);
     if (v != 1 && v != 2 && v != 4) return fail(PIKS_E_ARG, "PIKS_MULTI_RB must be 1, 2 or 4");
@@ -2474,17 +2535,50 @@ int flow_setup(Ctx& c) {
   if (c.nrhs != 1 || nc8 != 64 || c.stream_ordered || c.prm.precision != PIKS_PREC_FP64 || ns + 1 >= (1 << 28))
     return fail(PIKS_E_ARG, "PIKS_SWEEP_FLOW needs a single FP64 solve with 64 velocities per octant on a "
                             "context with its own stream");
-  // waves per octant: all resident at once (one-wave workgroups)
+  // waves per octant: all resident at once (one-wave workgroups).  The
+  // occupancy query is only an upper bound (it allowed 7 waves per SIMD where
+  // 6 were resident), so a probe launch of the sweep kernel itself, with the
+  // same registers, checks that every block of the grid is resident at once,
+  // and fewer waves are tried if not.
   int per_cu = 0, dev = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_flow, 64, 0));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
   const int64_t resident = (int64_t)per_cu * prop.multiProcessorCount;
-  if (c.flow_waves == 0) c.flow_waves = (int)std::min<int64_t>(resident / 8, 1024);
+  const bool forced = c.flow_waves != 0;
+  if (!forced) c.flow_waves = (int)std::min<int64_t>(resident / 8, 1024);
   if (c.flow_waves < 1 || 8 * (int64_t)c.flow_waves > resident)
     return fail(PIKS_E_ARG, "PIKS_SWEEP_FLOW: " + std::to_string(c.flow_waves) + " waves per octant do not fit the " +
                                 std::to_string(resident) + " resident waves");
+  int rc = 0;
+  if ((rc = dalloc(c, &c.flow_epoch, 64)) || (rc = dalloc(c, &c.flow_diag, 2 * 8 * (size_t)c.flow_waves)))
+    return rc;
+  // flow_epoch: the launch's state (k_flow_epoch)
+  c.flow_err = c.flow_epoch + 1;
+  for (int attempt = 0;; ++attempt) {
+    const unsigned nbp = 8u * (unsigned)c.flow_waves;
+    FlowArgs pa{};
+    pa.done = c.flow_epoch + 16;
+    pa.diag = c.flow_diag;
+    pa.probe = 1u << 14;
+    std::vector<uint32_t> seen(nbp, 0);
+    HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 64 * sizeof(uint32_t), c.stream));
+    hipLaunchKernelGGL(k_sweep_flow, dim3(nbp), dim3(64), 0, c.stream, pa, nullptr, nullptr, nullptr);
+    HIP_TRY(hipMemcpyAsync(seen.data(), c.flow_diag, nbp * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    // a block that gave up saw the arrivals up to then; the first to give up
+    // saw only blocks that were resident together (none had left yet), so
+    // the smallest count is what fits at once
+    uint32_t fit = 0xffffffffu;
+    for (uint32_t v : seen) fit = std::min(fit, v);
+    if (fit == 0xffffffffu) break;
+    const int w = (int)(fit / 8) - 8 * attempt;
+    if (forced || attempt == 3 || w < 8)
+      return fail(PIKS_E_HIP, "single-launch sweep: " + std::to_string(nbp) + " one-wave blocks are not resident "
+                              "at once (" + std::to_string(fit) + " were)");
+    c.flow_waves = std::min(w, c.flow_waves - 8);
+  }
   const unsigned nb = 8u * (unsigned)c.flow_waves;
   {
     uint32_t* probe = nullptr;
@@ -2534,20 +2628,31 @@ int flow_setup(Ctx& c) {
           if (sw[d1]) nbs[2 * ax] |= 1u << 31;
           else if (sw[d2]) nbs[2 * ax + 1] |= 1u << 31;
         }
+        uint32_t w7 = (uint32_t)w[7];
+        if (w7 >> 31) {  // reads wall slots the reference switched: resolve them here
+          const int64_t cell = w7 & 0x0fffffffu;
+          const int64_t dy = (int64_t)osy(o) * c.d.Nxt, dz = (int64_t)osz(o) * c.d.Nxyt;
+          const int64_t nbr[4] = {cell - dy, cell - 2 * dy, cell - dz, cell - 2 * dz};
+          for (int q = 0; q < 4; ++q) {
+            const int64_t nq = nbr[q];
+            if (nq < 0 || nq >= c.d.Ntotal) return fail(PIKS_E_GEOM, "single-launch sweep: a switched read "
+                                                                      "outside the grid");
+            const int64_t wq = flow_switched_word(g, o, nq, q < 2 ? 1 : 2);
+            if (wq == -1) continue;
+            if (wq < 0) return fail(PIKS_E_GEOM, "single-launch sweep: a switched read of a missing corner entry");
+            if (nbs[2 + q] >> 31) return fail(PIKS_E_GEOM, "single-launch sweep: a switched read of a swept slot");
+            nbs[2 + q] = (uint32_t)wq;
+          }
+          w7 &= 0x7fffffffu;
+        }
         for (int q = 0; q < 6; ++q) r.w[1 + q] = (int32_t)nbs[q];
-        r.w[7] = w[7];
+        r.w[7] = (int32_t)w7;
       }
   }
   c.flow_off[8] = at;
-  int rc = 0;
-  if ((rc = upload(c, &c.flow_rec, rec)) || (rc = dalloc(c, &c.flow_flags, (size_t)8 * (ns + 1))) ||
-      (rc = dalloc(c, &c.flow_epoch, 32)) || (rc = dalloc(c, &c.flow_diag, 2 * (size_t)nb)))
-    return rc;
-  // flow_epoch: [0] the epoch, [1..6] err (flag, then the first failure),
-  // [8..15] waves started per XCD, [16..23] tasks done per octant
-  c.flow_err = c.flow_epoch + 1;
+  if ((rc = upload(c, &c.flow_rec, rec)) || (rc = dalloc(c, &c.flow_flags, (size_t)8 * (ns + 1)))) return rc;
   HIP_TRY(hipMemsetAsync(c.flow_flags, 0, (size_t)8 * (ns + 1) * sizeof(uint32_t), c.stream));
-  HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 32 * sizeof(uint32_t), c.stream));
+  HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 64 * sizeof(uint32_t), c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));  // the host vectors above end here
   return 0;
 }
@@ -2710,7 +2815,11 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
       win[q].pad = 0;
     }
     if ((rc = upload(c, &c.hp_win, win))) return rc;
-    if (c.sweep_flow && (rc = flow_setup(c))) return rc;
+    if (c.flow_req != 0) {
+      const int r = flow_setup(c);
+      if (r == 0) c.sweep_flow = true;
+      else if (c.flow_req == 1) return r;
+    }
   }
   for (int F = 0; F < 4; ++F) {
     if ((rc = upload(c, &c.pos[F], g.pos[F]))) return rc;
@@ -3122,6 +3231,7 @@ int piks_get_info(void* ctx, piks_info* info) {
   info->precision = c.phase;
   info->steps_fp32 = c.dc_inner;
   info->dc_block = c.phase == PIKS_PREC_FP32_REFINED ? c.dc_len : 0;
+  info->sweep_path = c.sweep_flow ? 2 : c.sweep_tiles ? 1 : 0;
   return 0;
 }
 

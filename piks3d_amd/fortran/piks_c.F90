@@ -43,7 +43,7 @@ module piks_c
         integer(c_int32_t) :: precision
         integer(c_int64_t) :: steps_fp32
         integer(c_int32_t) :: dc_block
-        integer(c_int32_t) :: reserved
+        integer(c_int32_t) :: sweep_path
     end type piks_info
 
     interface

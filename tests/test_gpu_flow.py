@@ -1,9 +1,15 @@
-"""The single-launch sweep (PIKS_SWEEP_FLOW=1: the whole sweep of an
-iteration in one launch, octant o on XCD o, tasks waiting for their upwind
-neighbours' flags) computes every update with the same operations in the same
-order as the hyperplane launches, so f, fw, the moments and the check after
-every iteration equal the default path's bit for bit, in both reduction
-modes and with graph replays; it is refused where it does not apply."""
+"""The single-launch sweep (the whole sweep of an iteration in one launch,
+octant o on XCD o, tasks waiting for their upwind neighbours' flags; the
+default for FP64 single solves with 64 velocities per octant on a context
+with its own stream) computes every update with the same operations in the
+same order as the hyperplane launches (PIKS_SWEEP_FLOW=0).  So f, fw, the
+moments and the check after every iteration equal that path's bit for bit:
+in both reduction modes, with graph replays, with few waves per octant, and
+in the claims mode a launch falls back to when an octant's waves are not all
+resident (forced here from the host with a zero poll limit on the kernel's
+own bounded wait for them).  The sweep is refused where it does not apply,
+and the automatic choice leaves those contexts and the settings that shape
+the hyperplane sweep alone."""
 import numpy as np
 import pytest
 
@@ -11,6 +17,8 @@ from piks3d_amd import PiksError, Solver, images
 from piks3d_amd.params import Params
 
 pytestmark = pytest.mark.gpu
+
+_ENV = ("PIKS_SWEEP_FLOW", "PIKS_GRAPH", "PIKS_FLOW_WAVES", "PIKS_FLOW_BARRIER", "PIKS_OCT_GROUPS")
 
 
 def _pack(n, seed, nc=4):
@@ -20,28 +28,34 @@ def _pack(n, seed, nc=4):
     return p, img
 
 
-def _run(p, img, exact, env, monkeypatch, steps=4):
-    for k in ("PIKS_SWEEP_FLOW", "PIKS_GRAPH", "PIKS_FLOW_WAVES"):
+def _setenv(env, monkeypatch):
+    for k in _ENV:
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
+
+
+def _run(p, img, exact, env, monkeypatch, steps=4):
+    _setenv(env, monkeypatch)
     with Solver(p, img, precision="fp64", exact_order=exact) as s:
+        path = s.info.sweep_path
         per = []
         for _ in range(steps):
             s.iterate(1)
             per.append((np.stack(s.macro()), s.chk_converge()))
-        return s.f(), s.fw(), per
+        return s.f(), s.fw(), per, path
 
 
-@pytest.mark.parametrize("env", [{"PIKS_SWEEP_FLOW": "1"}, {"PIKS_SWEEP_FLOW": "1", "PIKS_GRAPH": "1"},
-                                 {"PIKS_SWEEP_FLOW": "1", "PIKS_FLOW_WAVES": "64"}],
-                         ids=lambda e: ",".join("%s=%s" % kv for kv in e.items()))
+@pytest.mark.parametrize("env", [{}, {"PIKS_SWEEP_FLOW": "1"}, {"PIKS_GRAPH": "1"}, {"PIKS_FLOW_WAVES": "64"},
+                                 {"PIKS_FLOW_BARRIER": "0"}],
+                         ids=lambda e: ",".join("%s=%s" % kv for kv in e.items()) or "default")
 @pytest.mark.parametrize("exact", [False, True])
 def test_flow_sweep_equals_hyperplane_launches(env, exact, monkeypatch):
     p, img = _pack(40, 5)
     print("40^3 pack, exact_order=%d, %s" % (exact, env), flush=True)
-    ref = _run(p, img, exact, {}, monkeypatch)
+    ref = _run(p, img, exact, {"PIKS_SWEEP_FLOW": "0"}, monkeypatch)
     got = _run(p, img, exact, env, monkeypatch)
+    assert ref[3] == 0 and got[3] == 2
     assert np.array_equal(got[0], ref[0])
     assert np.array_equal(got[1], ref[1])
     for (ma, ca), (mb, cb) in zip(got[2], ref[2]):
@@ -49,8 +63,19 @@ def test_flow_sweep_equals_hyperplane_launches(env, exact, monkeypatch):
         assert ca == cb
 
 
+def test_flow_sweep_on_a_larger_pack(monkeypatch):
+    # many tasks per wave (each of the waves takes tens of tasks per octant)
+    p, img = _pack(72, 11)
+    print("72^3 pack", flush=True)
+    ref = _run(p, img, False, {"PIKS_SWEEP_FLOW": "0"}, monkeypatch, steps=2)
+    got = _run(p, img, False, {}, monkeypatch, steps=2)
+    assert got[3] == 2
+    assert np.array_equal(got[0], ref[0])
+    assert got[2][-1][1] == ref[2][-1][1]
+
+
 def test_flow_sweep_refused_where_it_does_not_apply(monkeypatch):
-    monkeypatch.setenv("PIKS_SWEEP_FLOW", "1")
+    _setenv({"PIKS_SWEEP_FLOW": "1"}, monkeypatch)
     p, img = _pack(24, 3, nc=3)           # 27 velocities per octant
     p.exact_quadrature = True
     with pytest.raises(PiksError) as e:
@@ -59,3 +84,24 @@ def test_flow_sweep_refused_where_it_does_not_apply(monkeypatch):
     p, img = _pack(24, 3)
     with pytest.raises(PiksError):
         Solver(p, img, precision="fp32").close()
+
+
+def test_automatic_choice(monkeypatch):
+    # where the single launch does not apply, or a setting asks for the
+    # hyperplane sweep's shape, the automatic choice keeps the launches
+    p, img = _pack(24, 3, nc=3)
+    p.exact_quadrature = True
+    _setenv({}, monkeypatch)
+    with Solver(p, img, precision="fp64") as s:
+        assert s.info.sweep_path == 0
+    p, img = _pack(24, 3)
+    with Solver(p, img, precision="fp32") as s:
+        assert s.info.sweep_path == 0
+    with Solver(p, img, precision="fp64") as s:
+        assert s.info.sweep_path == 2
+    _setenv({"PIKS_OCT_GROUPS": "8"}, monkeypatch)
+    with Solver(p, img, precision="fp64") as s:
+        assert s.info.sweep_path == 0
+    _setenv({"PIKS_SWEEP_FLOW": "0"}, monkeypatch)
+    with Solver(p, img, precision="fp64") as s:
+        assert s.info.sweep_path == 0
