@@ -426,8 +426,8 @@ def c1_bench(args):
 
 
 def flow_compare_bench(args):
-    """The single-launch sweep (PIKS_SWEEP_FLOW=1) beside the default
-    hyperplane launches on sphere packs of the headline workload's kind
+    """The single-launch sweep (PIKS_SWEEP_FLOW=1, the default where it
+    applies) beside the hyperplane launches (PIKS_SWEEP_FLOW=0) on sphere packs of the headline workload's kind
     (porosity 0.40, radii 4-10, seed 2024, 4 fluid layers) at the sizes of
     --flow-sizes: sweep device time per iteration (mean over --steps
     iterations after --warmup untimed ones) and whether the two paths leave the
@@ -444,10 +444,7 @@ def flow_compare_bench(args):
                    accom=1.0)
         res = {}
         for mode in ("hyperplanes", "single_launch"):
-            if mode == "single_launch":
-                os.environ["PIKS_SWEEP_FLOW"] = "1"
-            else:
-                os.environ.pop("PIKS_SWEEP_FLOW", None)
+            os.environ["PIKS_SWEEP_FLOW"] = "1" if mode == "single_launch" else "0"
             with Solver(p, img, precision="fp64") as s:
                 s.iterate(max(1, args.warmup))
                 t = time.perf_counter()
@@ -609,6 +606,8 @@ def main():
     log("set-up %.1f s: Nfluid %d, walls %d, corners %d, device memory %.1f GB" % (
         time.perf_counter() - t, info.nfluid, info.nwall, info.ncorner, info.device_bytes / 1e9))
     nc = info.nc8 * 8
+    flow = info.sweep_path == 2
+    log("sweep: %s" % ("one launch per iteration (k_sweep_flow)" if flow else "one launch per hyperplane"))
     if args.warmup:
         s.iterate(args.warmup)
     log("warm-up done (%d steps)" % args.warmup)
@@ -643,8 +642,8 @@ def main():
     wl = "spheres200_phi040_kn1_v8"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(wl),
-            "kernel": "k_sweep1<double, false>",
+            "traffic": traffic_from_profiles(wl + ("_flow" if flow else "")),
+            "kernel": "k_sweep_flow" if flow else "k_sweep1<double, false>",
             "bytes_per_update": bytes_per_update,
             "sweep_ms_per_step": round(ms[0] / args.steps, 4)}
     cpu = None

@@ -2561,7 +2561,7 @@ int flow_setup(Ctx& c) {
     FlowArgs pa{};
     pa.done = c.flow_epoch + 16;
     pa.diag = c.flow_diag;
-    pa.probe = 1u << 14;
+    pa.probe = 1u << 12;  // about 1.4 ms; a launch places its blocks within tens of microseconds
     std::vector<uint32_t> seen(nbp, 0);
     HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 64 * sizeof(uint32_t), c.stream));
     hipLaunchKernelGGL(k_sweep_flow, dim3(nbp), dim3(64), 0, c.stream, pa, nullptr, nullptr, nullptr);
