@@ -105,3 +105,43 @@ def test_automatic_choice(monkeypatch):
     _setenv({"PIKS_SWEEP_FLOW": "0"}, monkeypatch)
     with Solver(p, img, precision="fp64") as s:
         assert s.info.sweep_path == 0
+
+
+def test_concurrent_single_launch_sweeps(monkeypatch):
+    # two contexts, each with its own stream, iterating at the same time from
+    # two host threads: their single launches compete for the CUs, so an
+    # octant whose waves are not all resident claims its tasks instead; each
+    # result equals the same context's run alone, bit for bit
+    import threading
+    _setenv({}, monkeypatch)
+    cases = [_pack(56, 21), _pack(48, 22)]
+    alone = []
+    for p, img in cases:
+        with Solver(p, img, precision="fp64") as s:
+            assert s.info.sweep_path == 2
+            s.iterate(6)
+            alone.append((s.f(), s.chk_converge()))
+    solvers = [Solver(p, img, precision="fp64") for p, img in cases]
+    errors = []
+
+    def work(s):
+        try:
+            for _ in range(6):
+                s.iterate(1)
+        except Exception as e:  # reported below, in the test's thread
+            errors.append(e)
+
+    try:
+        threads = [threading.Thread(target=work, args=(s,)) for s in solvers]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        assert not errors, errors
+        for s, (f, chk) in zip(solvers, alone):
+            assert s.info.sweep_path == 2
+            assert np.array_equal(s.f(), f)
+            assert s.chk_converge() == chk
+    finally:
+        for s in solvers:
+            s.close()
