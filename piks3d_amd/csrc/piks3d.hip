@@ -700,23 +700,40 @@ __device__ __forceinline__ void flow_poll3(FlowTask& t, uint32_t ep) {
   for (int ax = 0; ax < 3; ++ax) t.v[ax] = t.wt[ax] ? flow_poll(t.wt[ax]) : ep;
 }
 
-__device__ __forceinline__ void flow_issue(FlowTask& t, const TaskRec& r, const Mom* __restrict__ mom,
-                                           const double* fo, const uint32_t* flo, int lane, uint32_t ep) {
+// The flags a task waits for, from its record, and one poll of each.
+__device__ __forceinline__ void flow_targets(FlowTask& t, const TaskRec& r, const uint32_t* flo, uint32_t ep) {
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const uint32_t w1 = (uint32_t)r.w[1 + 2 * ax], w2 = (uint32_t)r.w[2 + 2 * ax];
+    t.wt[ax] = (w1 >> 28) == 8u   ? flo + (w1 & 0x0fffffffu)
+               : (w2 >> 28) == 8u ? flo + (w2 & 0x0fffffffu)
+                                  : nullptr;
+  }
+  flow_poll3(t, ep);
+}
+
+// The rest of a task's set-up: record fields, moments, previous value.
+__device__ __forceinline__ void flow_fields(FlowTask& t, const TaskRec& r, const Mom* __restrict__ mom,
+                                            const double* fo, int lane) {
   t.self = (uint32_t)r.w[0] & 0x0fffffffu;
   t.cell = (uint32_t)r.w[7] & 0x0fffffffu;
   t.flg = (uint32_t)r.w[7] >> 28;
 #pragma unroll
   for (int q = 0; q < 6; ++q) t.nbs[q] = (uint32_t)r.w[1 + q];
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    const uint32_t w1 = t.nbs[2 * ax], w2 = t.nbs[2 * ax + 1];
-    t.wt[ax] = (w1 >> 28) == 8u   ? flo + (w1 & 0x0fffffffu)
-               : (w2 >> 28) == 8u ? flo + (w2 & 0x0fffffffu)
-                                  : nullptr;
-  }
   t.m = ld_const(mom + t.cell);
   t.fp = fo[(size_t)t.self * 64 + lane];
-  flow_poll3(t, ep);
+}
+
+__device__ __forceinline__ void flow_issue(FlowTask& t, const TaskRec& r, const Mom* __restrict__ mom,
+                                           const double* fo, const uint32_t* flo, int lane, uint32_t ep) {
+  flow_fields(t, r, mom, fo, lane);
+  flow_targets(t, r, flo, ep);
+}
+
+// Whether the polls read the sweep's epoch on every flag the task waits for.
+__device__ __forceinline__ bool flow_ready(const FlowTask& t, uint32_t ep) {
+  return __builtin_amdgcn_readfirstlane(t.v[0]) == ep && __builtin_amdgcn_readfirstlane(t.v[1]) == ep &&
+         __builtin_amdgcn_readfirstlane(t.v[2]) == ep;
 }
 
 // The first velocity of an upwind source word of a flow record: bits 0..27
@@ -774,23 +791,16 @@ __device__ __forceinline__ bool flow_wait(FlowTask& t, uint32_t ep, uint32_t* er
   return false;
 }
 
-// One wave: its tasks k = w, w + W, ... of its XCD's octant, pipelined: after
-// task k's stores, the loads of task k + W (flow_issue, with its first poll)
-// and the record of task k + 2W go out before the wait for those stores, so
-// the stores' acknowledgement, the next task's set-up and its first poll
-// cost one memory round trip.  Task k's flag is set before any poll of task
-// k + W is looked at again, so a wave never waits for its own task.
-// One wave: its tasks k = w, w + W, ... of its XCD's octant, pipelined:
-// after task k's stores, the loads of task k + W (flow_issue, with its first
-// poll) and the record of task k + 2W go out before the wait for those
-// stores, so the stores' acknowledgement, the next task's set-up and its
-// first poll cost one memory round trip.  Task k's flag is set before any
-// poll of task k + W is looked at again, so a wave never waits for its own
-// task.
+// One wave: its tasks of its XCD's octant (k = w, w + W, ... or claimed),
+// pipelined so that a task costs one memory round trip besides its update:
+// the next task's flags are polled while this task computes, and after this
+// task's stores its moments, previous value and, if those flags were set,
+// its upwind values go out before the wait for the stores.  This task's
+// flag is set before any poll of the next task is looked at again, so a wave
+// never waits for its own task.
 // The records, moments and coefficients, which nothing in the launch writes,
-// come through restrict arguments: their loads are then scalar loads and the
-// kernel needs 61 VGPRs (8 waves per SIMD; 96 with them in vector registers
-// and the switched reads resolved on the device).
+// come through restrict arguments, so their loads are scalar loads and the
+// kernel runs 6 waves per SIMD (SGPR-bound; its occupancy query says 7).
 __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __restrict__ rec,
                                                    const Mom* __restrict__ mom, const SweepCoef* __restrict__ coef) {
   const int lane = threadIdx.x;
@@ -831,20 +841,25 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __
   const bool px = osx((int)o) > 0, py = osy((int)o) > 0, pz = osz((int)o) > 0;
   uint32_t* const next = a.next + o;
   // static: tasks wv, wv + W, ... (a wave beyond W has none); claims: the
-  // next unclaimed task, claimed one task ahead
+  // next unclaimed task
   int64_t k = claims ? (int64_t)__builtin_amdgcn_readfirstlane(flow_claim(next, lane)) : wv < a.waves ? wv : n;
   if (k < n) {
+    // task k: its record's fields, moments, previous value and upwind values
     FlowTask t;
     flow_issue(t, ld_const(rec + base + k), mom, fo, flo, lane, ep);
+    if (!flow_wait(t, ep, a.err, a.spin, o, (uint32_t)k, flo)) goto out;
+    asm volatile("" ::: "memory");  // the upwind loads stay after the polls that cleared them
+    double nb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nb[q] = flow_source(a, o, t.nbs[q])[lane];
     int64_t kn = claims ? (int64_t)__builtin_amdgcn_readfirstlane(flow_claim(next, lane)) : k + a.waves;
     TaskRec rn = ld_const(rec + base + (kn < n ? kn : k));
     while (true) {
-      if (!flow_wait(t, ep, a.err, a.spin, o, (uint32_t)k, flo)) break;
-      asm volatile("" ::: "memory");  // the upwind loads stay after the polls that cleared them
-      double nb[6];
-#pragma unroll
-      for (int q = 0; q < 6; ++q) nb[q] = flow_source(a, o, t.nbs[q])[lane];
-      // the coefficients (eight entries, cache-resident) with the upwind values
+      const bool more = kn < n;
+      // the next task's flags, polled while this task computes
+      FlowTask tn;
+      if (more) flow_targets(tn, rn, flo, ep);
+      // the coefficients (eight entries, cache-resident)
       const SweepCoef cf = ld_const(coef + (t.flg & 7u));
       // sweep1_task's update, operation for operation
       const double mx = px ? t.m.x : -t.m.x;
@@ -865,6 +880,9 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __
       den = den + cyl * cf.dy;
       den = den + czl * cf.dz;
       const double fnew = num / den;
+      // the next task's polls, read before this task's stores go out (a
+      // wait on them after the stores would wait for the stores too)
+      const bool early = more && flow_ready(tn, ep);
       const uint32_t self = t.self, cell = t.cell;
       fo[(size_t)self * 64 + lane] = fnew;
       if (a.part) {
@@ -875,22 +893,42 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __
           a.part[((size_t)cell * 8 + o) * 4 + q] = zz;
         }
       }
-      const bool more = kn < n;
+      // the next task's loads go out with this task's stores: its upwind
+      // values too when its flags were already set (it then needs no memory
+      // round trip of its own before its update)
       uint32_t knn = 0;
       if (more) {
-        flow_issue(t, rn, mom, fo, flo, lane, ep);
+        flow_fields(tn, rn, mom, fo, lane);
+        if (early) {
+          asm volatile("" ::: "memory");  // the upwind loads stay after the polls that cleared them
+#pragma unroll
+          for (int q = 0; q < 6; ++q) nb[q] = flow_source(a, o, tn.nbs[q])[lane];
+        }
         if (claims) knn = flow_claim(next, lane);
+        else rn = ld_const(rec + base + (kn + a.waves < n ? kn + a.waves : kn));
       }
       // the slot's values (and partials) reach L2 before its flag
       __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
       if (lane == 0) __hip_atomic_store(flo + self, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       ++done;
       if (!more) break;
+      t = tn;
       k = kn;
-      kn = claims ? (int64_t)__builtin_amdgcn_readfirstlane(knn) : kn + a.waves;
-      rn = ld_const(rec + base + (kn < n ? kn : k));
+      if (claims) {
+        kn = (int64_t)__builtin_amdgcn_readfirstlane(knn);
+        rn = ld_const(rec + base + (kn < n ? kn : k));
+      } else {
+        kn += a.waves;
+      }
+      if (!early) {  // the next task waits on this one's or others' flags
+        if (!flow_wait(t, ep, a.err, a.spin, o, (uint32_t)k, flo)) break;
+        asm volatile("" ::: "memory");  // the upwind loads stay after the polls that cleared them
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nb[q] = flow_source(a, o, t.nbs[q])[lane];
+      }
     }
   }
+out:
   if (lane == 0) {
     a.diag[gridDim.x + blockIdx.x] = done;
     __hip_atomic_fetch_add(a.done + o, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
