@@ -726,8 +726,21 @@ __device__ __forceinline__ void flow_fields(FlowTask& t, const TaskRec& r, const
 
 __device__ __forceinline__ void flow_issue(FlowTask& t, const TaskRec& r, const Mom* __restrict__ mom,
                                            const double* fo, const uint32_t* flo, int lane, uint32_t ep) {
-  flow_fields(t, r, mom, fo, lane);
-  flow_targets(t, r, flo, ep);
+  t.self = (uint32_t)r.w[0] & 0x0fffffffu;
+  t.cell = (uint32_t)r.w[7] & 0x0fffffffu;
+  t.flg = (uint32_t)r.w[7] >> 28;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) t.nbs[q] = (uint32_t)r.w[1 + q];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const uint32_t w1 = t.nbs[2 * ax], w2 = t.nbs[2 * ax + 1];
+    t.wt[ax] = (w1 >> 28) == 8u   ? flo + (w1 & 0x0fffffffu)
+               : (w2 >> 28) == 8u ? flo + (w2 & 0x0fffffffu)
+                                  : nullptr;
+  }
+  t.m = ld_const(mom + t.cell);
+  t.fp = fo[(size_t)t.self * 64 + lane];
+  flow_poll3(t, ep);
 }
 
 // Whether the polls read the sweep's epoch on every flag the task waits for.
@@ -801,6 +814,7 @@ __device__ __forceinline__ bool flow_wait(FlowTask& t, uint32_t ep, uint32_t* er
 // The records, moments and coefficients, which nothing in the launch writes,
 // come through restrict arguments, so their loads are scalar loads and the
 // kernel runs 6 waves per SIMD (SGPR-bound; its occupancy query says 7).
+template <bool kEarly>
 __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __restrict__ rec,
                                                    const Mom* __restrict__ mom, const SweepCoef* __restrict__ coef) {
   const int lane = threadIdx.x;
@@ -840,6 +854,69 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __
   const double cxl = a.cx[lane], cyl = a.cy[lane], czl = a.cz[lane], wl = a.w[lane];
   const bool px = osx((int)o) > 0, py = osy((int)o) > 0, pz = osz((int)o) > 0;
   uint32_t* const next = a.next + o;
+  if constexpr (!kEarly) {
+  // static: tasks wv, wv + W, ... (a wave beyond W has none); claims: the
+  // next unclaimed task, claimed one task ahead
+  int64_t k = claims ? (int64_t)__builtin_amdgcn_readfirstlane(flow_claim(next, lane)) : wv < a.waves ? wv : n;
+  if (k < n) {
+    FlowTask t;
+    flow_issue(t, ld_const(rec + base + k), mom, fo, flo, lane, ep);
+    int64_t kn = claims ? (int64_t)__builtin_amdgcn_readfirstlane(flow_claim(next, lane)) : k + a.waves;
+    TaskRec rn = ld_const(rec + base + (kn < n ? kn : k));
+    while (true) {
+      if (!flow_wait(t, ep, a.err, a.spin, o, (uint32_t)k, flo)) break;
+      asm volatile("" ::: "memory");  // the upwind loads stay after the polls that cleared them
+      double nb[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) nb[q] = flow_source(a, o, t.nbs[q])[lane];
+      // the coefficients (eight entries, cache-resident) with the upwind values
+      const SweepCoef cf = ld_const(coef + (t.flg & 7u));
+      // sweep1_task's update, operation for operation
+      const double mx = px ? t.m.x : -t.m.x;
+      const double my = py ? t.m.y : -t.m.y;
+      const double mz = pz ? t.m.z : -t.m.z;
+      double tt = cxl * mx;
+      tt = tt + cyl * my;
+      tt = tt + czl * mz;
+      const double feq = wl * (t.m.r + 1.0 * tt);
+      double num = a.mu0 * (feq - 0.5 * t.fp);
+      num = num + cxl * cf.x1 * nb[0];
+      num = num + cxl * cf.x2 * nb[1];
+      num = num + cyl * cf.y1 * nb[2];
+      num = num + cyl * cf.y2 * nb[3];
+      num = num + czl * cf.z1 * nb[4];
+      num = num + czl * cf.z2 * nb[5];
+      double den = 0.5 * a.mu0 + cxl * cf.dx;
+      den = den + cyl * cf.dy;
+      den = den + czl * cf.dz;
+      const double fnew = num / den;
+      const uint32_t self = t.self, cell = t.cell;
+      fo[(size_t)self * 64 + lane] = fnew;
+      if (a.part) {
+        const double z = wave_sum4(fnew, cxl * fnew, cyl * fnew, czl * fnew);
+        if ((lane & 15) == 0) {
+          const uint32_t q = (uint32_t)lane >> 4;
+          const double zz = ((kOctNeg >> (4 * o + q)) & 1u) ? -z : z;
+          a.part[((size_t)cell * 8 + o) * 4 + q] = zz;
+        }
+      }
+      const bool more = kn < n;
+      uint32_t knn = 0;
+      if (more) {
+        flow_issue(t, rn, mom, fo, flo, lane, ep);
+        if (claims) knn = flow_claim(next, lane);
+      }
+      // the slot's values (and partials) reach L2 before its flag
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+      if (lane == 0) __hip_atomic_store(flo + self, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      ++done;
+      if (!more) break;
+      k = kn;
+      kn = claims ? (int64_t)__builtin_amdgcn_readfirstlane(knn) : kn + a.waves;
+      rn = ld_const(rec + base + (kn < n ? kn : k));
+    }
+  }
+  } else {
   // static: tasks wv, wv + W, ... (a wave beyond W has none); claims: the
   // next unclaimed task
   int64_t k = claims ? (int64_t)__builtin_amdgcn_readfirstlane(flow_claim(next, lane)) : wv < a.waves ? wv : n;
@@ -927,6 +1004,7 @@ __global__ __launch_bounds__(64) void k_sweep_flow(FlowArgs a, const TaskRec* __
         for (int q = 0; q < 6; ++q) nb[q] = flow_source(a, o, t.nbs[q])[lane];
       }
     }
+  }
   }
 out:
   if (lane == 0) {
@@ -2119,7 +2197,8 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
       FlowCounts fc{};
       for (int q = 0; q < 8; ++q) fc.n[q] = (uint32_t)(c.flow_off[q + 1] - c.flow_off[q]);
       hipLaunchKernelGGL(k_flow_epoch, dim3(1), dim3(64), 0, c.stream, c.flow_epoch);
-      hipLaunchKernelGGL(k_sweep_flow, dim3(8u * (unsigned)c.flow_waves), dim3(64), 0, c.stream, fa, fa.rec,
+      hipLaunchKernelGGL(c.flow_early ? k_sweep_flow<true> : k_sweep_flow<false>, dim3(8u * (unsigned)c.flow_waves),
+                         dim3(64), 0, c.stream, fa, fa.rec,
                          fa.mom, fa.coef);
       hipLaunchKernelGGL(k_flow_done, dim3(1), dim3(64), 0, c.stream, c.flow_epoch, fc);
     }
@@ -2476,6 +2555,7 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
     c->flow_waves = v;
   }
   if (const char* b = std::getenv("PIKS_FLOW_BARRIER")) c->flow_barrier = std::atoi(b);
+  if (const char* b = std::getenv("PIKS_FLOW_EARLY")) c->flow_early_req = std::atoi(b) != 0 ? 1 : 0;
   if (const char* b = std::getenv("PIKS_MULTI_RB")) {
     const int v = std::atoi(b);
     if (v != 1 && v != 2 && v != 4) return fail(PIKS_E_ARG, "PIKS_MULTI_RB must be 1, 2 or 4");
@@ -2567,6 +2647,8 @@ size_t ctx_device_bytes(const Ctx& c) {
 // octant o's tasks in hyperplane order, words 1..6 the neighbours' slot
 // indices with bit 31 set on the nearest swept neighbour along each axis
 // (the one the task waits for).
+int64_t tasks_per_hyperplane(const Geometry& g);
+
 int flow_setup(Ctx& c) {
   const Geometry& g = c.g;
   const int64_t nc8 = c.d.Nc8, ns = g.nstore;
@@ -2578,8 +2660,14 @@ int flow_setup(Ctx& c) {
   // 6 were resident), so a probe launch of the sweep kernel itself, with the
   // same registers, checks that every block of the grid is resident at once,
   // and fewer waves are tried if not.
+  // Two loop structures (k_sweep_flow<kEarly>), each with its own registers:
+  // the early upwind loads pay where an octant's hyperplanes average at least
+  // two tasks per wave (measured: 40^3 and 64^3 packs faster without, 100^3
+  // and 200^3 with), so the choice follows W, and the probe is repeated for
+  // the kernel that will run.
+  using FlowKernel = void (*)(FlowArgs, const TaskRec*, const Mom*, const SweepCoef*);
   int per_cu = 0, dev = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_flow, 64, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_flow<false>, 64, 0));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
@@ -2594,29 +2682,35 @@ int flow_setup(Ctx& c) {
     return rc;
   // flow_epoch: the launch's state (k_flow_epoch)
   c.flow_err = c.flow_epoch + 1;
-  for (int attempt = 0;; ++attempt) {
-    const unsigned nbp = 8u * (unsigned)c.flow_waves;
-    FlowArgs pa{};
-    pa.done = c.flow_epoch + 16;
-    pa.diag = c.flow_diag;
-    pa.probe = 1u << 12;  // about 1.4 ms; a launch places its blocks within tens of microseconds
-    std::vector<uint32_t> seen(nbp, 0);
-    HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 64 * sizeof(uint32_t), c.stream));
-    hipLaunchKernelGGL(k_sweep_flow, dim3(nbp), dim3(64), 0, c.stream, pa, nullptr, nullptr, nullptr);
-    HIP_TRY(hipMemcpyAsync(seen.data(), c.flow_diag, nbp * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
-    HIP_TRY(hipStreamSynchronize(c.stream));
-    // a block that gave up saw the arrivals up to then; the first to give up
-    // saw only blocks that were resident together (none had left yet), so
-    // the smallest count is what fits at once
-    uint32_t fit = 0xffffffffu;
-    for (uint32_t v : seen) fit = std::min(fit, v);
-    if (fit == 0xffffffffu) break;
-    const int w = (int)(fit / 8) - 8 * attempt;
-    if (forced || attempt == 3 || w < 8)
-      return fail(PIKS_E_HIP, "single-launch sweep: " + std::to_string(nbp) + " one-wave blocks are not resident "
-                              "at once (" + std::to_string(fit) + " were)");
-    c.flow_waves = std::min(w, c.flow_waves - 8);
-  }
+  auto probe_waves = [&](FlowKernel kern) -> int {
+    for (int attempt = 0;; ++attempt) {
+      const unsigned nbp = 8u * (unsigned)c.flow_waves;
+      FlowArgs pa{};
+      pa.done = c.flow_epoch + 16;
+      pa.diag = c.flow_diag;
+      pa.probe = 1u << 12;  // about 1.4 ms; a launch places its blocks within tens of microseconds
+      std::vector<uint32_t> seen(nbp, 0);
+      HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 64 * sizeof(uint32_t), c.stream));
+      hipLaunchKernelGGL(kern, dim3(nbp), dim3(64), 0, c.stream, pa, nullptr, nullptr, nullptr);
+      HIP_TRY(hipMemcpyAsync(seen.data(), c.flow_diag, nbp * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+      HIP_TRY(hipStreamSynchronize(c.stream));
+      // a block that gave up saw the arrivals up to then; the first to give
+      // up saw only blocks that were resident together (none had left yet),
+      // so the smallest count is what fits at once
+      uint32_t fit = 0xffffffffu;
+      for (uint32_t v : seen) fit = std::min(fit, v);
+      if (fit == 0xffffffffu) return 0;
+      const int w = (int)(fit / 8) - 8 * attempt;
+      if (forced || attempt == 3 || w < 8)
+        return fail(PIKS_E_HIP, "single-launch sweep: " + std::to_string(nbp) + " one-wave blocks are not "
+                                "resident at once (" + std::to_string(fit) + " were)");
+      c.flow_waves = std::min(w, c.flow_waves - 8);
+    }
+  };
+  if ((rc = probe_waves(k_sweep_flow<false>))) return rc;
+  if (c.flow_early_req >= 0) c.flow_early = c.flow_early_req != 0;
+  else c.flow_early = tasks_per_hyperplane(g) / 8 >= 2 * (int64_t)c.flow_waves;
+  if (c.flow_early && (rc = probe_waves(k_sweep_flow<true>))) return rc;
   const unsigned nb = 8u * (unsigned)c.flow_waves;
   {
     uint32_t* probe = nullptr;

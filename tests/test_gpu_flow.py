@@ -7,7 +7,9 @@ moments and the check after every iteration equal that path's bit for bit:
 in both reduction modes, with graph replays, with few waves per octant, and
 in the claims mode a launch falls back to when an octant's waves are not all
 resident (forced here from the host with a zero poll limit on the kernel's
-own bounded wait for them).  The sweep is refused where it does not apply,
+own bounded wait for them), and with the next task's upwind loads issued
+early (with the current task's stores; by default only for images whose
+hyperplanes hold at least two tasks per wave).  The sweep is refused where it does not apply,
 and the automatic choice leaves those contexts and the settings that shape
 the hyperplane sweep alone."""
 import numpy as np
@@ -18,7 +20,7 @@ from piks3d_amd.params import Params
 
 pytestmark = pytest.mark.gpu
 
-_ENV = ("PIKS_SWEEP_FLOW", "PIKS_GRAPH", "PIKS_FLOW_WAVES", "PIKS_FLOW_BARRIER", "PIKS_OCT_GROUPS")
+_ENV = ("PIKS_SWEEP_FLOW", "PIKS_GRAPH", "PIKS_FLOW_WAVES", "PIKS_FLOW_BARRIER", "PIKS_FLOW_EARLY", "PIKS_OCT_GROUPS")
 
 
 def _pack(n, seed, nc=4):
@@ -47,7 +49,9 @@ def _run(p, img, exact, env, monkeypatch, steps=4):
 
 
 @pytest.mark.parametrize("env", [{}, {"PIKS_SWEEP_FLOW": "1"}, {"PIKS_GRAPH": "1"}, {"PIKS_FLOW_WAVES": "64"},
-                                 {"PIKS_FLOW_BARRIER": "0"}],
+                                 {"PIKS_FLOW_BARRIER": "0"}, {"PIKS_FLOW_EARLY": "1"},
+                                 {"PIKS_FLOW_EARLY": "1", "PIKS_FLOW_WAVES": "64"},
+                                 {"PIKS_FLOW_EARLY": "1", "PIKS_FLOW_BARRIER": "0"}],
                          ids=lambda e: ",".join("%s=%s" % kv for kv in e.items()) or "default")
 @pytest.mark.parametrize("exact", [False, True])
 def test_flow_sweep_equals_hyperplane_launches(env, exact, monkeypatch):
@@ -63,12 +67,13 @@ def test_flow_sweep_equals_hyperplane_launches(env, exact, monkeypatch):
         assert ca == cb
 
 
-def test_flow_sweep_on_a_larger_pack(monkeypatch):
+@pytest.mark.parametrize("early", ["0", "1"])
+def test_flow_sweep_on_a_larger_pack(early, monkeypatch):
     # many tasks per wave (each of the waves takes tens of tasks per octant)
     p, img = _pack(72, 11)
-    print("72^3 pack", flush=True)
+    print("72^3 pack, early loads %s" % early, flush=True)
     ref = _run(p, img, False, {"PIKS_SWEEP_FLOW": "0"}, monkeypatch, steps=2)
-    got = _run(p, img, False, {}, monkeypatch, steps=2)
+    got = _run(p, img, False, {"PIKS_FLOW_EARLY": early}, monkeypatch, steps=2)
     assert got[3] == 2
     assert np.array_equal(got[0], ref[0])
     assert got[2][-1][1] == ref[2][-1][1]
