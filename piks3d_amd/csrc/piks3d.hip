@@ -590,32 +590,35 @@ __global__ __launch_bounds__(64) void k_sweep1_rhs(SweepArgs<T> a) { sweep1_task
 // ------------------------------------------------------------------ single-launch sweep
 
 // The whole sweep of one source iteration in one launch: no launch gaps
-// between hyperplanes (opt-in: PIKS_SWEEP_FLOW=1).  A block sweeps the
-// octant numbered like the XCD it runs on (its hardware XCD id), so every
-// producer and consumer of an octant's slots share one L2, and takes its
-// wave number w in that octant from a per-XCD counter.  Wave w takes the
-// octant's tasks w, w + W, w + 2W, ... in hyperplane order (rec, built by
-// the host): a task's upwind neighbours come earlier in that order, so with
-// every wave of the grid resident the earliest unfinished task can always
-// proceed and no wait is circular.  That needs the GPU to itself: W is sized
-// from the occupancy, and kernels of other streams holding CUs could keep
-// waves of the grid from starting.  (Claiming tasks from a per-octant
-// counter instead, which needs no residency, measured 8-12x slower: the
-// claims serialise at 60-90 ns on the counter's address.)  A finished task
-// sets its slot's flag to the sweep's epoch after its stores have reached
-// L2; a task waits for the flags of the nearest swept upwind neighbour
-// along each axis (their own waits cover the one behind).  Flags are polled
-// with agent-scope loads, which miss the CU's L1 (an L1 line polled before
-// the flag changed would be served again: a workgroup-scope invalidate
-// leaves it).  Values are plain loads served by the XCD's L2: within the
-// launch no CU loads a slot's lines before they are final (its readers wait
-// for its flag or for the flag of a task that waited for it, and its own
-// task's read of the previous value is followed by its store, which updates
-// that CU's L1), and the L1 starts the launch invalidated.  Every wait gives
-// up after `spin` polls, and a wave beyond W on its XCD (an uneven deal of
-// blocks) stops at once: either sets err and the wave ends, so the grid
-// always drains; k_flow_done then checks that every octant's tasks were
-// done.  The update is sweep1_task's, operation for operation.
+// between hyperplanes (the default for FP64 single solves with 64
+// velocities per octant; PIKS_SWEEP_FLOW).  A block sweeps the octant
+// numbered like the XCD it runs on (its hardware XCD id), so every producer
+// and consumer of an octant's slots share one L2, and takes its arrival
+// number w in that octant from a per-XCD counter.  Tasks are in hyperplane
+// order (rec, built by the host), so a task's upwind neighbours come earlier.
+// When all W waves of an octant have arrived (the set-up probe sizes W to
+// what is resident at once), wave w takes tasks w, w + W, w + 2W, ...: the
+// earliest unfinished task can always proceed and no wait is circular.  If
+// a wave waits `barrier` polls without that happening (kernels of other
+// streams holding CUs), the octant switches to claims: every wave, late
+// ones included, takes the next task from a per-octant counter, which needs
+// no residency (a task waits only on tasks claimed before it by running
+// waves) but serialises the claims (8-12x slower when every task is
+// claimed).  A finished task sets its slot's flag to the sweep's epoch after
+// its stores have reached L2; a task waits for the flags of the nearest
+// swept upwind neighbour along each axis (their own waits cover the one
+// behind).  Flags are polled with agent-scope loads, which miss the CU's L1
+// (an L1 line polled before the flag changed would be served again: a
+// workgroup-scope invalidate leaves it).  Values are plain loads served by
+// the XCD's L2: within the launch no CU loads a slot's lines before they
+// are final (its readers wait for its flag or for the flag of a task that
+// waited for it, and its own task's read of the previous value is followed
+// by its store, which updates that CU's L1), and the L1 starts the launch
+// invalidated.  Every wait gives up after `spin` polls (a safety net: it
+// sets err and the wave ends, so the grid always drains); k_flow_done then
+// checks that every octant's tasks were done.  The update is sweep1_task's,
+// operation for operation; reads of switched wall slots come resolved in
+// the records (flow_switched_word).
 struct FlowArgs {
   double* f;
   const Mom* mom;
@@ -2642,11 +2645,13 @@ size_t ctx_device_bytes(const Ctx& c) {
 
 // The single-launch sweep's schedule and state, refused unless the context
 // can use it: FP64 single solve, 64 velocities per octant, its own stream,
-// slot indices below 2^28, and W blocks on each of the eight XCDs (a probe
-// launch of the same grid reads each block's XCD id).  Records:
-// octant o's tasks in hyperplane order, words 1..6 the neighbours' slot
-// indices with bit 31 set on the nearest swept neighbour along each axis
-// (the one the task waits for).
+// slot indices below 2^28, W waves per octant resident at once (probe
+// launches of the kernel itself) and W blocks on each of the eight XCDs (a
+// probe launch of the same grid reads each block's XCD id).  Records:
+// octant o's tasks in hyperplane order; words 1..6 the upwind sources
+// (flow_source): the neighbours' slot indices, with bit 31 set on the
+// nearest swept neighbour along each axis (the one the task waits for), and
+// reads of wall slots the reference switched resolved to their source.
 int64_t tasks_per_hyperplane(const Geometry& g);
 
 int flow_setup(Ctx& c) {
