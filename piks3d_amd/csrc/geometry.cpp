@@ -725,4 +725,78 @@ int64_t flow_switched_word(const Geometry& g, int o, int64_t n, int axis) {
   return (corner * 8 + o) | 0x90000000ll;
 }
 
+void make_window_relative(Geometry& g, int nc8, int threads) {
+  parallel_for(g.nhp, threads, [&](int h) {
+    for (int64_t t = g.hp_off[(size_t)h]; t < g.hp_off[(size_t)h + 1]; ++t) {
+      int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
+      const int o = (int)((uint32_t)w[0] >> 28);
+      const int64_t lo = g.hp_lo[(size_t)h * 8 + o], len = g.hp_len[(size_t)h * 8 + o];
+      auto rel = [&](int64_t p) { return (p == g.nstore ? len : p - lo) * nc8; };
+      w[0] = (int32_t)((uint32_t)rel((uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
+      for (int q = 1; q <= 6; ++q) w[q] = (int32_t)rel(w[q]);
+    }
+  });
+}
+
+int build_flow_records(const Geometry& g, int nc8, std::vector<int32_t>& rec, int64_t off[9], std::string& err) {
+  const int64_t ns = g.nstore;
+  // storage index of a window-relative element offset (the zero slot: ns)
+  auto slot_of = [&](int64_t h, int o, uint32_t rel) -> uint32_t {
+    const int64_t lo = g.hp_lo[(size_t)h * 8 + o], len = g.hp_len[(size_t)h * 8 + o];
+    return (int64_t)rel == len * nc8 ? (uint32_t)ns : (uint32_t)(rel / nc8 + lo);
+  };
+  std::vector<uint8_t> swept((size_t)8 * (ns + 1), 0);
+  for (int o = 0; o < 8; ++o)
+    for (int h = 0; h < g.nhp; ++h)
+      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
+        const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
+        swept[(size_t)o * (ns + 1) + slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu)] = 1;
+      }
+  rec.assign((size_t)g.hp_off[(size_t)g.nhp] * Geometry::kTaskWords, 0);
+  int64_t at = 0;
+  for (int o = 0; o < 8; ++o) {
+    off[o] = at;
+    const uint8_t* sw = &swept[(size_t)o * (ns + 1)];
+    for (int h = 0; h < g.nhp; ++h)
+      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
+        const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
+        int32_t* r = &rec[(size_t)(at++ * Geometry::kTaskWords)];
+        r[0] = (int32_t)(slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
+        uint32_t nbs[6];
+        for (int q = 0; q < 6; ++q) nbs[q] = slot_of(h, o, (uint32_t)w[1 + q]);
+        for (int ax = 0; ax < 3; ++ax) {
+          const uint32_t d1 = nbs[2 * ax], d2 = nbs[2 * ax + 1];
+          if (sw[d1]) nbs[2 * ax] |= 1u << 31;
+          else if (sw[d2]) nbs[2 * ax + 1] |= 1u << 31;
+        }
+        uint32_t w7 = (uint32_t)w[7];
+        if (w7 >> 31) {  // reads wall slots the reference switched: resolve them here
+          const int64_t cell = w7 & 0x0fffffffu;
+          const int64_t dy = (int64_t)osy(o) * g.Nxt, dz = (int64_t)osz(o) * g.Nxyt;
+          const int64_t nbr[4] = {cell - dy, cell - 2 * dy, cell - dz, cell - 2 * dz};
+          for (int q = 0; q < 4; ++q) {
+            const int64_t nq = nbr[q];
+            if (nq < 0 || nq >= g.Ntotal) {
+              err = "single-launch sweep: a switched read outside the grid";
+              return PIKS_E_GEOM;
+            }
+            const int64_t wq = flow_switched_word(g, o, nq, q < 2 ? 1 : 2);
+            if (wq == -1) continue;
+            if (wq < 0 || (nbs[2 + q] >> 31)) {
+              err = wq < 0 ? "single-launch sweep: a switched read of a missing corner entry"
+                           : "single-launch sweep: a switched read of a swept slot";
+              return PIKS_E_GEOM;
+            }
+            nbs[2 + q] = (uint32_t)wq;
+          }
+          w7 &= 0x7fffffffu;
+        }
+        for (int q = 0; q < 6; ++q) r[1 + q] = (int32_t)nbs[q];
+        r[7] = (int32_t)w7;
+      }
+  }
+  off[8] = at;
+  return 0;
+}
+
 }  // namespace piks

@@ -161,6 +161,22 @@ constexpr int tile_rec_xhalo(int ns, int q) { return 1 + ns + q; }
 constexpr int tile_rec_yhalo(int ns, int ty, int q) { return 1 + ns + 2 * ty + q; }
 constexpr int tile_rec_cell(int ns, int tx, int ty, int s) { return 1 + ns + 2 * ty + 2 * tx + s; }
 
+// The hyperplane sweep's task records as its kernels read them (in place):
+// element offsets inside the (hyperplane, octant) storage window, with nc8
+// velocities per octant, the zero slot as the window's end.
+void make_window_relative(Geometry& g, int nc8, int threads);
+
+// Records of the single-launch sweep, from the window-relative hyperplane
+// records: the tasks in octant-major, hyperplane order, kTaskWords words
+// each (off[o] the first task of octant o, off[8] the count).  Word 0 is the
+// task's storage index; words 1..6 its upwind sources (flow_switched_word's
+// encoding; bit 31 with bits 28..30 clear marks the nearest upwind
+// neighbour along each axis that the octant sweeps, whose flag the task
+// waits for); word 7 the cell and flags, bit 31 cleared once the task's
+// reads of switched wall slots are resolved into words 3..6.  Returns 0, or
+// PIKS_E_GEOM with err filled.
+int build_flow_records(const Geometry& g, int nc8, std::vector<int32_t>& rec, int64_t off[9], std::string& err);
+
 // Source word of a single-launch sweep record for the read of neighbour n
 // (a y neighbour for axis 1, z for axis 2) by a task of octant o, when the
 // reference switched that wall slot (the views of Geometry::wall_views, read

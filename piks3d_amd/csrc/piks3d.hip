@@ -2605,18 +2605,7 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
     return fail(PIKS_E_ARG, "a hyperplane's storage window exceeds 2 GiB (image or velocity grid too large)");
   // task records as the sweep reads them: element offsets inside the
   // (hyperplane, octant) window, the zero slot as the window's end
-  Geometry& gm = c->g;
-  const int64_t nc8 = c->d.Nc8;
-  parallel_for(gm.nhp, host_threads(), [&](int h) {
-    for (int64_t t = gm.hp_off[(size_t)h]; t < gm.hp_off[(size_t)h + 1]; ++t) {
-      int32_t* w = &gm.hp_task[(size_t)(t * Geometry::kTaskWords)];
-      const int o = (int)((uint32_t)w[0] >> 28);
-      const int64_t lo = gm.hp_lo[(size_t)h * 8 + o], len = gm.hp_len[(size_t)h * 8 + o];
-      auto rel = [&](int64_t p) { return (p == gm.nstore ? len : p - lo) * nc8; };
-      w[0] = (int32_t)((uint32_t)rel((uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
-      for (int q = 1; q <= 6; ++q) w[q] = (int32_t)rel(w[q]);
-    }
-  });
+  make_window_relative(c->g, c->d.Nc8, host_threads());
   *out = c.release();
   return 0;
 }
@@ -2736,58 +2725,13 @@ int flow_setup(Ctx& c) {
                                     " of the probe's " + std::to_string(nb) + " blocks, not " +
                                     std::to_string(c.flow_waves));
   }
-  // slot index of a window-relative element offset (the zero slot: ns)
-  std::vector<uint8_t> swept((size_t)8 * (ns + 1), 0);
-  auto slot_of = [&](int64_t h, int o, uint32_t rel) -> uint32_t {
-    const int64_t lo = g.hp_lo[(size_t)h * 8 + o], len = g.hp_len[(size_t)h * 8 + o];
-    return (int64_t)rel == len * nc8 ? (uint32_t)ns : (uint32_t)(rel / nc8 + lo);
-  };
-  for (int o = 0; o < 8; ++o)
-    for (int h = 0; h < g.nhp; ++h)
-      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
-        const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
-        swept[(size_t)o * (ns + 1) + slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu)] = 1;
-      }
-  std::vector<TaskRec> rec((size_t)g.hp_off[(size_t)g.nhp]);
-  int64_t at = 0;
-  for (int o = 0; o < 8; ++o) {
-    c.flow_off[o] = at;
-    const uint8_t* sw = &swept[(size_t)o * (ns + 1)];
-    for (int h = 0; h < g.nhp; ++h)
-      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
-        const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
-        TaskRec& r = rec[(size_t)at++];
-        r.w[0] = (int32_t)(slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
-        uint32_t nbs[6];
-        for (int q = 0; q < 6; ++q) nbs[q] = slot_of(h, o, (uint32_t)w[1 + q]);
-        for (int ax = 0; ax < 3; ++ax) {
-          const uint32_t d1 = nbs[2 * ax], d2 = nbs[2 * ax + 1];
-          if (sw[d1]) nbs[2 * ax] |= 1u << 31;
-          else if (sw[d2]) nbs[2 * ax + 1] |= 1u << 31;
-        }
-        uint32_t w7 = (uint32_t)w[7];
-        if (w7 >> 31) {  // reads wall slots the reference switched: resolve them here
-          const int64_t cell = w7 & 0x0fffffffu;
-          const int64_t dy = (int64_t)osy(o) * c.d.Nxt, dz = (int64_t)osz(o) * c.d.Nxyt;
-          const int64_t nbr[4] = {cell - dy, cell - 2 * dy, cell - dz, cell - 2 * dz};
-          for (int q = 0; q < 4; ++q) {
-            const int64_t nq = nbr[q];
-            if (nq < 0 || nq >= c.d.Ntotal) return fail(PIKS_E_GEOM, "single-launch sweep: a switched read "
-                                                                      "outside the grid");
-            const int64_t wq = flow_switched_word(g, o, nq, q < 2 ? 1 : 2);
-            if (wq == -1) continue;
-            if (wq < 0) return fail(PIKS_E_GEOM, "single-launch sweep: a switched read of a missing corner entry");
-            if (nbs[2 + q] >> 31) return fail(PIKS_E_GEOM, "single-launch sweep: a switched read of a swept slot");
-            nbs[2 + q] = (uint32_t)wq;
-          }
-          w7 &= 0x7fffffffu;
-        }
-        for (int q = 0; q < 6; ++q) r.w[1 + q] = (int32_t)nbs[q];
-        r.w[7] = (int32_t)w7;
-      }
-  }
-  c.flow_off[8] = at;
-  if ((rc = upload(c, &c.flow_rec, rec)) || (rc = dalloc(c, &c.flow_flags, (size_t)8 * (ns + 1)))) return rc;
+  std::vector<int32_t> rec;
+  std::string err;
+  if ((rc = build_flow_records(g, (int)nc8, rec, c.flow_off, err))) return fail(rc, err);
+  static_assert(sizeof(TaskRec) == Geometry::kTaskWords * sizeof(int32_t), "one record per task");
+  if ((rc = dalloc(c, &c.flow_rec, rec.size() / Geometry::kTaskWords))) return rc;
+  HIP_TRY(hipMemcpyAsync(c.flow_rec, rec.data(), rec.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+  if ((rc = dalloc(c, &c.flow_flags, (size_t)8 * (ns + 1)))) return rc;
   HIP_TRY(hipMemsetAsync(c.flow_flags, 0, (size_t)8 * (ns + 1) * sizeof(uint32_t), c.stream));
   HIP_TRY(hipMemsetAsync(c.flow_epoch, 0, 64 * sizeof(uint32_t), c.stream));
   HIP_TRY(hipStreamSynchronize(c.stream));  // the host vectors above end here
