@@ -2198,7 +2198,7 @@ int enqueue_t(Ctx& c, State<T>& st, int nsteps, int ev0, const IterMode& im) {
       for (int q = 0; q < 9; ++q) fa.off[q] = c.flow_off[q];
       fa.ostride = d.ostride;
       fa.nslot = g.nstore + 1;
-      fa.spin = 1u << 20;
+      fa.spin = 1u << 20;  // about 1-2 s of polls 16 x 64 cycles apart, far beyond any wait of a sweep
       fa.diag = c.flow_diag;
       FlowCounts fc{};
       for (int q = 0; q < 8; ++q) fc.n[q] = (uint32_t)(c.flow_off[q + 1] - c.flow_off[q]);
