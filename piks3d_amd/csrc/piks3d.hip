@@ -2903,6 +2903,7 @@ int ctx_create_device(Ctx& c, hipStream_t stream) {
       const int r = flow_setup(c);
       if (r == 0) c.sweep_flow = true;
       else if (c.flow_req == 1) return r;
+      else set_error(0, "");  // automatic choice: the hyperplane launches serve, and no error stays behind
     }
   }
   for (int F = 0; F < 4; ++F) {

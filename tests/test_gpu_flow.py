@@ -15,7 +15,7 @@ the hyperplane sweep alone."""
 import numpy as np
 import pytest
 
-from piks3d_amd import PiksError, Solver, images
+from piks3d_amd import PiksError, Solver, images, lib
 from piks3d_amd.params import Params
 
 pytestmark = pytest.mark.gpu
@@ -102,6 +102,8 @@ def test_automatic_choice(monkeypatch):
     p, img = _pack(24, 3)
     with Solver(p, img, precision="fp32") as s:
         assert s.info.sweep_path == 0
+        # the automatic choice's refusal is not an error and leaves no message
+        assert not lib().piks_last_error()
     with Solver(p, img, precision="fp64") as s:
         assert s.info.sweep_path == 2
     _setenv({"PIKS_OCT_GROUPS": "8"}, monkeypatch)
