@@ -122,6 +122,7 @@ struct Ctx {
   int flow_waves = 0;
   int flow_barrier = 1 << 12;         // polls for an octant's waves to arrive
   int flow_early_req = -1;            // early upwind loads: -1 by size, 0 off, 1 on
+  int flow_strips = 1;                // bands of y an octant sweeps one after another
   bool flow_early = false;
   TaskRec* flow_rec = nullptr;        // octant-major tasks (flow_setup)
   int64_t flow_off[9] = {};

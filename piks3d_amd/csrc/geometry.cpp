@@ -738,7 +738,8 @@ void make_window_relative(Geometry& g, int nc8, int threads) {
   });
 }
 
-int build_flow_records(const Geometry& g, int nc8, std::vector<int32_t>& rec, int64_t off[9], std::string& err) {
+int build_flow_records(const Geometry& g, int nc8, std::vector<int32_t>& rec, int64_t off[9], std::string& err,
+                       int strips) {
   const int64_t ns = g.nstore;
   // storage index of a window-relative element offset (the zero slot: ns)
   auto slot_of = [&](int64_t h, int o, uint32_t rel) -> uint32_t {
@@ -753,47 +754,69 @@ int build_flow_records(const Geometry& g, int nc8, std::vector<int32_t>& rec, in
         swept[(size_t)o * (ns + 1) + slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu)] = 1;
       }
   rec.assign((size_t)g.hp_off[(size_t)g.nhp] * Geometry::kTaskWords, 0);
+  if (strips < 1) strips = 1;
+  const int64_t nyt = g.Nxyt / g.Nxt, band = (nyt + strips - 1) / strips;
+  std::vector<std::pair<int, int64_t>> order;  // (hyperplane, task)
+  std::vector<int64_t> first((size_t)strips + 1);
   int64_t at = 0;
   for (int o = 0; o < 8; ++o) {
     off[o] = at;
     const uint8_t* sw = &swept[(size_t)o * (ns + 1)];
+    // strip of a task: its cell's band of y, counted along the octant's y
+    // direction (upwind neighbours lie in the same band or an earlier one)
+    auto strip_of = [&](int64_t t) {
+      const int64_t cell = (uint32_t)g.hp_task[(size_t)(t * Geometry::kTaskWords) + 7] & 0x0fffffffu;
+      const int64_t y = (cell % g.Nxyt) / g.Nxt;
+      return (int)((osy(o) > 0 ? y : nyt - 1 - y) / band);
+    };
+    // stable counting sort of the hyperplane order by strip
+    std::fill(first.begin(), first.end(), 0);
     for (int h = 0; h < g.nhp; ++h)
-      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
-        const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
-        int32_t* r = &rec[(size_t)(at++ * Geometry::kTaskWords)];
-        r[0] = (int32_t)(slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
-        uint32_t nbs[6];
-        for (int q = 0; q < 6; ++q) nbs[q] = slot_of(h, o, (uint32_t)w[1 + q]);
-        for (int ax = 0; ax < 3; ++ax) {
-          const uint32_t d1 = nbs[2 * ax], d2 = nbs[2 * ax + 1];
-          if (sw[d1]) nbs[2 * ax] |= 1u << 31;
-          else if (sw[d2]) nbs[2 * ax + 1] |= 1u << 31;
-        }
-        uint32_t w7 = (uint32_t)w[7];
-        if (w7 >> 31) {  // reads wall slots the reference switched: resolve them here
-          const int64_t cell = w7 & 0x0fffffffu;
-          const int64_t dy = (int64_t)osy(o) * g.Nxt, dz = (int64_t)osz(o) * g.Nxyt;
-          const int64_t nbr[4] = {cell - dy, cell - 2 * dy, cell - dz, cell - 2 * dz};
-          for (int q = 0; q < 4; ++q) {
-            const int64_t nq = nbr[q];
-            if (nq < 0 || nq >= g.Ntotal) {
-              err = "single-launch sweep: a switched read outside the grid";
-              return PIKS_E_GEOM;
-            }
-            const int64_t wq = flow_switched_word(g, o, nq, q < 2 ? 1 : 2);
-            if (wq == -1) continue;
-            if (wq < 0 || (nbs[2 + q] >> 31)) {
-              err = wq < 0 ? "single-launch sweep: a switched read of a missing corner entry"
-                           : "single-launch sweep: a switched read of a swept slot";
-              return PIKS_E_GEOM;
-            }
-            nbs[2 + q] = (uint32_t)wq;
-          }
-          w7 &= 0x7fffffffu;
-        }
-        for (int q = 0; q < 6; ++q) r[1 + q] = (int32_t)nbs[q];
-        r[7] = (int32_t)w7;
+      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t)
+        ++first[(size_t)strip_of(t) + 1];
+    for (int s = 0; s < strips; ++s) first[(size_t)s + 1] += first[(size_t)s];
+    order.assign((size_t)first[(size_t)strips], {0, 0});
+    for (int h = 0; h < g.nhp; ++h)
+      for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t)
+        order[(size_t)first[(size_t)strip_of(t)]++] = {h, t};
+    for (const auto& ht : order) {
+      const int h = ht.first;
+      const int64_t t = ht.second;
+      const int32_t* w = &g.hp_task[(size_t)(t * Geometry::kTaskWords)];
+      int32_t* r = &rec[(size_t)(at++ * Geometry::kTaskWords)];
+      r[0] = (int32_t)(slot_of(h, o, (uint32_t)w[0] & 0x0fffffffu) | (uint32_t)o << 28);
+      uint32_t nbs[6];
+      for (int q = 0; q < 6; ++q) nbs[q] = slot_of(h, o, (uint32_t)w[1 + q]);
+      for (int ax = 0; ax < 3; ++ax) {
+        const uint32_t d1 = nbs[2 * ax], d2 = nbs[2 * ax + 1];
+        if (sw[d1]) nbs[2 * ax] |= 1u << 31;
+        else if (sw[d2]) nbs[2 * ax + 1] |= 1u << 31;
       }
+      uint32_t w7 = (uint32_t)w[7];
+      if (w7 >> 31) {  // reads wall slots the reference switched: resolve them here
+        const int64_t cell = w7 & 0x0fffffffu;
+        const int64_t dy = (int64_t)osy(o) * g.Nxt, dz = (int64_t)osz(o) * g.Nxyt;
+        const int64_t nbr[4] = {cell - dy, cell - 2 * dy, cell - dz, cell - 2 * dz};
+        for (int q = 0; q < 4; ++q) {
+          const int64_t nq = nbr[q];
+          if (nq < 0 || nq >= g.Ntotal) {
+            err = "single-launch sweep: a switched read outside the grid";
+            return PIKS_E_GEOM;
+          }
+          const int64_t wq = flow_switched_word(g, o, nq, q < 2 ? 1 : 2);
+          if (wq == -1) continue;
+          if (wq < 0 || (nbs[2 + q] >> 31)) {
+            err = wq < 0 ? "single-launch sweep: a switched read of a missing corner entry"
+                         : "single-launch sweep: a switched read of a swept slot";
+            return PIKS_E_GEOM;
+          }
+          nbs[2 + q] = (uint32_t)wq;
+        }
+        w7 &= 0x7fffffffu;
+      }
+      for (int q = 0; q < 6; ++q) r[1 + q] = (int32_t)nbs[q];
+      r[7] = (int32_t)w7;
+    }
   }
   off[8] = at;
   return 0;

@@ -168,14 +168,19 @@ void make_window_relative(Geometry& g, int nc8, int threads);
 
 // Records of the single-launch sweep, from the window-relative hyperplane
 // records: the tasks in octant-major, hyperplane order, kTaskWords words
-// each (off[o] the first task of octant o, off[8] the count).  Word 0 is the
+// each (off[o] the first task of octant o, off[8] the count).  With
+// strips > 1 an octant's tasks go strip by strip (strips equal bands of y,
+// in the octant's y direction), in hyperplane order within a strip: every
+// task still follows its upwind neighbours, and the tasks written and not
+// yet read at any time span two hyperplanes of one strip, not of the image.  Word 0 is the
 // task's storage index; words 1..6 its upwind sources (flow_switched_word's
 // encoding; bit 31 with bits 28..30 clear marks the nearest upwind
 // neighbour along each axis that the octant sweeps, whose flag the task
 // waits for); word 7 the cell and flags, bit 31 cleared once the task's
 // reads of switched wall slots are resolved into words 3..6.  Returns 0, or
 // PIKS_E_GEOM with err filled.
-int build_flow_records(const Geometry& g, int nc8, std::vector<int32_t>& rec, int64_t off[9], std::string& err);
+int build_flow_records(const Geometry& g, int nc8, std::vector<int32_t>& rec, int64_t off[9], std::string& err,
+                       int strips = 1);
 
 // Source word of a single-launch sweep record for the read of neighbour n
 // (a y neighbour for axis 1, z for axis 2) by a task of octant o, when the

@@ -2562,6 +2562,10 @@ int ctx_create_host(const piks_params* p, const uint8_t* raw, int64_t nbytes, Ct
   }
   if (const char* b = std::getenv("PIKS_FLOW_BARRIER")) c->flow_barrier = std::atoi(b);
   if (const char* b = std::getenv("PIKS_FLOW_EARLY")) c->flow_early_req = std::atoi(b) != 0 ? 1 : 0;
+  if (const char* b = std::getenv("PIKS_FLOW_STRIPS")) {
+    c->flow_strips = std::atoi(b);
+    if (c->flow_strips < 1 || c->flow_strips > 64) return fail(PIKS_E_ARG, "PIKS_FLOW_STRIPS must be 1..64");
+  }
   if (const char* b = std::getenv("PIKS_MULTI_RB")) {
     const int v = std::atoi(b);
     if (v != 1 && v != 2 && v != 4) return fail(PIKS_E_ARG, "PIKS_MULTI_RB must be 1, 2 or 4");
@@ -2730,7 +2734,7 @@ int flow_setup(Ctx& c) {
   }
   std::vector<int32_t> rec;
   std::string err;
-  if ((rc = build_flow_records(g, (int)nc8, rec, c.flow_off, err))) return fail(rc, err);
+  if ((rc = build_flow_records(g, (int)nc8, rec, c.flow_off, err, c.flow_strips))) return fail(rc, err);
   static_assert(sizeof(TaskRec) == Geometry::kTaskWords * sizeof(int32_t), "one record per task");
   if ((rc = dalloc(c, &c.flow_rec, rec.size() / Geometry::kTaskWords))) return rc;
   HIP_TRY(hipMemcpyAsync(c.flow_rec, rec.data(), rec.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));

@@ -1,14 +1,16 @@
 // Host check of the single-launch sweep's records (build_flow_records) on one
 // image, against the hyperplane schedule they are made from.
 //
-//   flow_records_check <raw image> <nx> <ny> <nz> <fluid layers> <wall order> [plant]
+//   flow_records_check <raw image> <nx> <ny> <nz> <fluid layers> <wall order> [plant] [strips=N]
 //
 // With "plant" the records are given two mistakes before the checks (one
 // task's wait mark cleared, one switched read pointed at the next slot),
-// which the checks must report.
+// which the checks must report.  "strips=N" builds the records strip by
+// strip (N bands of y).
 // The schedule is built by build_geometry; a copy of its task records before
 // make_window_relative (storage indices) is the reference.  Checks:
-//  * every task of octant o appears once, in hyperplane order;
+//  * every task of octant o appears once (in hyperplane order unless the
+//    records go strip by strip);
 //  * each axis waits on the nearest upwind neighbour the octant sweeps
 //    (the first of the two that is swept), and only on it;
 //  * every upwind source of the octant's own storage that the octant sweeps
@@ -34,11 +36,21 @@
 using namespace piks;
 
 int main(int argc, char** argv) {
-  if (argc != 7 && argc != 8) {
-    std::fprintf(stderr, "usage: %s raw nx ny nz layers wallorder [plant]\n", argv[0]);
+  if (argc < 7 || argc > 9) {
+    std::fprintf(stderr, "usage: %s raw nx ny nz layers wallorder [plant] [strips=N]\n", argv[0]);
     return 2;
   }
-  const bool plant = argc == 8 && std::string(argv[7]) == "plant";
+  bool plant = false;
+  int strips = 1;
+  for (int a = 7; a < argc; ++a) {
+    const std::string s = argv[a];
+    if (s == "plant") plant = true;
+    else if (s.rfind("strips=", 0) == 0) strips = std::atoi(s.c_str() + 7);
+    else {
+      std::fprintf(stderr, "unknown argument %s\n", argv[a]);
+      return 2;
+    }
+  }
   std::ifstream fh(argv[1], std::ios::binary);
   std::vector<uint8_t> raw((std::istreambuf_iterator<char>(fh)), {});
   GeomInput in{std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atoi(argv[6])};
@@ -54,7 +66,7 @@ int main(int argc, char** argv) {
   make_window_relative(g, nc8, 4);
   std::vector<int32_t> rec;
   int64_t off[9];
-  if (build_flow_records(g, nc8, rec, off, err)) {
+  if (build_flow_records(g, nc8, rec, off, err, strips)) {
     std::printf("records: %s\n", err.c_str());
     return 1;
   }
@@ -83,15 +95,32 @@ int main(int argc, char** argv) {
   std::vector<std::vector<int64_t>> prod(8, std::vector<int64_t>((size_t)ns + 1, -1));
   std::vector<std::vector<std::vector<int64_t>>> deps(8);
   for (int o = 0; o < 8; ++o) {
-    // the octant's tasks in hyperplane order, from the plain schedule
-    std::vector<int64_t> order;
-    std::vector<uint8_t> swept((size_t)ns + 1, 0);
+    // the octant's tasks in the records' order: each record's plain task (the
+    // one that writes the same slot), every plain task of the octant once
+    std::vector<int64_t> order, plain_order, task_of((size_t)ns + 1, -1);
+    std::vector<uint8_t> swept((size_t)ns + 1, 0), seen((size_t)ns + 1, 0);
     for (int h = 0; h < g.nhp; ++h)
       for (int64_t t = g.hp_oct[(size_t)h * 8 + o]; t < g.hp_oct[(size_t)h * 8 + o + 1]; ++t) {
-        order.push_back(t);
-        swept[(size_t)((uint32_t)plain[(size_t)t * 8] & 0x0fffffffu)] = 1;
+        const uint32_t s = (uint32_t)plain[(size_t)t * 8] & 0x0fffffffu;
+        plain_order.push_back(t);
+        swept[(size_t)s] = 1;
+        task_of[(size_t)s] = t;
       }
-    if ((int64_t)order.size() != off[o + 1] - off[o]) fail("octant task count", o, (int64_t)order.size());
+    if ((int64_t)plain_order.size() != off[o + 1] - off[o]) fail("octant task count", o, (int64_t)plain_order.size());
+    for (int64_t k = 0; k < off[o + 1] - off[o]; ++k) {
+      const uint32_t s = (uint32_t)rec[(size_t)(off[o] + k) * 8] & 0x0fffffffu;
+      if (s >= (uint32_t)ns || task_of[s] < 0 || seen[s]) {
+        fail("slot not a task of the octant, or twice", o, k);
+        continue;
+      }
+      seen[s] = 1;
+      order.push_back(task_of[s]);
+    }
+    if (order.size() != plain_order.size()) {
+      deps[o].resize((size_t)(off[o + 1] - off[o]));
+      continue;
+    }
+    if (strips == 1 && order != plain_order) fail("hyperplane order", o, 0);
     for (size_t k = 0; k < order.size(); ++k)
       prod[o][(size_t)((uint32_t)plain[order[k] * 8] & 0x0fffffffu)] = (int64_t)k;
     deps[o].resize(order.size());

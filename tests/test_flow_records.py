@@ -7,7 +7,8 @@ waits on the nearest swept upwind neighbour per axis, every swept source
 updated by an earlier task of the octant, every switched read addressed as
 the hyperplane kernels' switched_source addresses it (src/solver.F90:184-204
 and analogues decide those switches), and the static split over 8, 64 and
-768 waves per octant drains.  The same program with mistakes planted in the
+768 waves per octant drains, also when the records go strip by strip
+(bands of y, one after another).  The same program with mistakes planted in the
 records must fail."""
 import os
 import shutil
@@ -49,9 +50,11 @@ def _run(checker, tmp_path, name, make, layers, *extra):
                           capture_output=True, text=True, timeout=300)
 
 
+@pytest.mark.parametrize("strips", [1, 3])
 @pytest.mark.parametrize("name,make,layers", CASES, ids=[c[0] for c in CASES])
-def test_flow_records(checker, tmp_path, name, make, layers):
-    r = _run(checker, tmp_path, name, make, layers)
+def test_flow_records(checker, tmp_path, name, make, layers, strips):
+    # strips > 1: each octant's tasks band of y by band of y
+    r = _run(checker, tmp_path, name, make, layers, "strips=%d" % strips)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "bad 0" in r.stdout

@@ -20,7 +20,8 @@ from piks3d_amd.params import Params
 
 pytestmark = pytest.mark.gpu
 
-_ENV = ("PIKS_SWEEP_FLOW", "PIKS_GRAPH", "PIKS_FLOW_WAVES", "PIKS_FLOW_BARRIER", "PIKS_FLOW_EARLY", "PIKS_OCT_GROUPS")
+_ENV = ("PIKS_SWEEP_FLOW", "PIKS_GRAPH", "PIKS_FLOW_WAVES", "PIKS_FLOW_BARRIER", "PIKS_FLOW_EARLY", "PIKS_FLOW_STRIPS",
+        "PIKS_OCT_GROUPS")
 
 
 def _pack(n, seed, nc=4):
@@ -51,7 +52,8 @@ def _run(p, img, exact, env, monkeypatch, steps=4):
 @pytest.mark.parametrize("env", [{}, {"PIKS_SWEEP_FLOW": "1"}, {"PIKS_GRAPH": "1"}, {"PIKS_FLOW_WAVES": "64"},
                                  {"PIKS_FLOW_BARRIER": "0"}, {"PIKS_FLOW_EARLY": "1"},
                                  {"PIKS_FLOW_EARLY": "1", "PIKS_FLOW_WAVES": "64"},
-                                 {"PIKS_FLOW_EARLY": "1", "PIKS_FLOW_BARRIER": "0"}],
+                                 {"PIKS_FLOW_EARLY": "1", "PIKS_FLOW_BARRIER": "0"}, {"PIKS_FLOW_STRIPS": "3"},
+                                 {"PIKS_FLOW_STRIPS": "5", "PIKS_FLOW_EARLY": "1", "PIKS_FLOW_WAVES": "64"}],
                          ids=lambda e: ",".join("%s=%s" % kv for kv in e.items()) or "default")
 @pytest.mark.parametrize("exact", [False, True])
 def test_flow_sweep_equals_hyperplane_launches(env, exact, monkeypatch):
@@ -67,13 +69,14 @@ def test_flow_sweep_equals_hyperplane_launches(env, exact, monkeypatch):
         assert ca == cb
 
 
-@pytest.mark.parametrize("early", ["0", "1"])
-def test_flow_sweep_on_a_larger_pack(early, monkeypatch):
-    # many tasks per wave (each of the waves takes tens of tasks per octant)
+@pytest.mark.parametrize("early,strips", [("0", "1"), ("1", "1"), ("1", "4")])
+def test_flow_sweep_on_a_larger_pack(early, strips, monkeypatch):
+    # many tasks per wave (each of the waves takes tens of tasks per octant);
+    # with strips, each octant's tasks go band of y by band of y
     p, img = _pack(72, 11)
-    print("72^3 pack, early loads %s" % early, flush=True)
+    print("72^3 pack, early loads %s, strips %s" % (early, strips), flush=True)
     ref = _run(p, img, False, {"PIKS_SWEEP_FLOW": "0"}, monkeypatch, steps=2)
-    got = _run(p, img, False, {"PIKS_FLOW_EARLY": early}, monkeypatch, steps=2)
+    got = _run(p, img, False, {"PIKS_FLOW_EARLY": early, "PIKS_FLOW_STRIPS": strips}, monkeypatch, steps=2)
     assert got[3] == 2
     assert np.array_equal(got[0], ref[0])
     assert got[2][-1][1] == ref[2][-1][1]
