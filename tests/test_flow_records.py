@@ -50,10 +50,11 @@ def _run(checker, tmp_path, name, make, layers, *extra):
                           capture_output=True, text=True, timeout=300)
 
 
-@pytest.mark.parametrize("strips", [1, 3])
+@pytest.mark.parametrize("strips", [1, 3, 64])
 @pytest.mark.parametrize("name,make,layers", CASES, ids=[c[0] for c in CASES])
 def test_flow_records(checker, tmp_path, name, make, layers, strips):
-    # strips > 1: each octant's tasks band of y by band of y
+    # strips > 1: each octant's tasks band of y by band of y (64: more bands
+    # than the 40^3 image has rows, one row per band)
     r = _run(checker, tmp_path, name, make, layers, "strips=%d" % strips)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
